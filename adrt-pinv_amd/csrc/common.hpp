@@ -1,0 +1,94 @@
+// Shared definitions of the ADRT HIP library: complex arithmetic, error reporting, launch checks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace adrt {
+
+// ---------------------------------------------------------------- complex values in registers / LDS / HBM
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+    T x, y;
+};
+using c64 = cplx<float>;
+using c128 = cplx<double>;
+
+template <typename T>
+__host__ __device__ inline cplx<T> mk(T x, T y) { return cplx<T>{x, y}; }
+template <typename T>
+__host__ __device__ inline cplx<T> add(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ inline cplx<T> sub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ inline cplx<T> mul(cplx<T> a, cplx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T>
+__host__ __device__ inline cplx<T> scale(T s, cplx<T> a) { return {s * a.x, s * a.y}; }
+template <typename T>
+__host__ __device__ inline cplx<T> conj(cplx<T> a) { return {a.x, -a.y}; }
+// conj(a) * b
+template <typename T>
+__host__ __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
+template <typename T, typename U>
+__host__ __device__ inline cplx<T> cast(cplx<U> a) { return {(T)a.x, (T)a.y}; }
+
+// ---------------------------------------------------------------- error reporting (thread-local last error)
+void set_error(const std::string& msg);
+const char* last_error_cstr();
+
+enum Status : int {
+    OK = 0,
+    E_INVALID = -1,  // bad argument (size, level range, null pointer, op list)
+    E_HIP = -2,      // HIP runtime error
+    E_NOMEM = -3,    // device memory budget exceeded or allocation failed
+    E_STATE = -4,    // object used in the wrong state (e.g. launching an uncaptured graph)
+};
+
+#define ADRT_HIP(call)                                                                                      \
+    do {                                                                                                    \
+        hipError_t _e = (call);                                                                             \
+        if (_e != hipSuccess) {                                                                             \
+            ::adrt::set_error(std::string(#call " failed: ") + hipGetErrorString(_e) + " at " __FILE__ ":" + \
+                              std::to_string(__LINE__));                                                    \
+            return ::adrt::E_HIP;                                                                           \
+        }                                                                                                   \
+    } while (0)
+
+#define ADRT_REQUIRE(cond, msg)                       \
+    do {                                              \
+        if (!(cond)) {                                \
+            ::adrt::set_error(std::string(msg));      \
+            return ::adrt::E_INVALID;                 \
+        }                                             \
+    } while (0)
+
+#define ADRT_TRY(expr)              \
+    do {                            \
+        int _s = (expr);            \
+        if (_s != ::adrt::OK) return _s; \
+    } while (0)
+
+inline int launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("launch of ") + what + " failed: " + hipGetErrorString(e));
+        return E_HIP;
+    }
+    return OK;
+}
+
+// ---------------------------------------------------------------- row operations
+enum OpKind : int { OP_B = 0, OP_BT = 1, OP_P = 2 };
+constexpr int MAX_OPS = 8;
+struct RowOp {
+    int kind, lo, hi;
+};
+
+// Precision of the row engine.  F64: every row op in double.  F32: butterflies, Chebyshev steps and the
+// offset FFTs in float (compensated accumulation of the Chebyshev iterate), level 1, the dense table and
+// everything in image space in double.
+enum Precision : int { PREC_F64 = 0, PREC_F32 = 1 };
+
+}  // namespace adrt

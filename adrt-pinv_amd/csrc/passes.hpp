@@ -1,0 +1,43 @@
+// Packed-pass kernels around the row engine (offset FFTs, quadrant packing / unpacking) and the image-space
+// kernels of the outer iteration (G_1 stencil, lorr projector, Chebyshev updates).
+#pragma once
+#include "common.hpp"
+
+namespace adrt {
+
+// Forward FFT over offsets of the packed quadrant strips (fastops.py:209-214).
+//   mode 0 (image):   strip_q = T_q x zero-padded to L rows (fastspec.py:142-152)
+//   mode 1 (data):    strip_q = iota(b_q): offset h in [-(N-1), N) -> row h mod L (fastspec.py:170-174)
+// Output rows: Y[((s*2 + pair)*L + k)*N + col], pair 0 = q0 + i q1, pair 1 = q2 + i q3.
+int launch_pack_fft(int n, int mode, const double* src, void* Y, int S, int prec, const void* tw, hipStream_t st);
+
+// Inverse FFT over offsets and split of the packed pairs into the four real quadrant strips, rows h < N only
+// (the window keeps nothing else, fastops.py:216-220): Sq[((s*4 + q)*N + h)*N + col].
+int launch_ifft_unpack(int n, const void* Z, void* Sq, int S, int prec, const void* tw, hipStream_t st);
+
+// out[s] = sum_q TqT(q, Sq[s, q]) windowed to N x N (fastspec.py:154-168), summed as (q0 + q1) + (q2 + q3).
+// If y != nullptr the kernel writes y - out instead (the outer residual).
+int launch_combine(int n, const void* Sq, int prec, const double* y, double* out, int S, hipStream_t st);
+
+// Twiddle table exp(sign * 2 pi i m / L), m < L/2, in the precision's type (host builds it once per plan).
+size_t twiddle_bytes(int n, int prec);
+
+// Image-space kernels on S images of N x N doubles.
+int launch_scale(const double* x, double alpha, double* out, long long count, hipStream_t st);
+// out = x + w1 * (a * r) + w0 * (x - xp)          (fastops.py:255-256; fasthybrid.py:123-124); xp == null means 0
+int launch_cheb_update(const double* x, const double* xp, const double* r, double a, double w1, double w0,
+                       double* out, long long count, hipStream_t st);
+// out = x + w1 * (a * (y - G1 x)) + w0 * (x - xp) (one step of g1_inv, fasthybrid.py:95-98, 114-126); xp == null
+// means the zero start of the first step.
+int launch_g1_step(int n, const double* y, const double* x, const double* xp, double a, double w1, double w0,
+                   double* out, int S, hipStream_t st);
+// out = (y - Pi y)/4 + Pi y / 12 (fasthybrid.py:101-111)
+int launch_lorr(int n, const double* y, double* out, int S, hipStream_t st);
+
+// Forward / transpose ADRT (adrtcore.py:63-137).  dtype 0: int32 (transpose accumulates into int64),
+// 1: float, 2: double.  work holds one (S, 4, 2N-1, N) array of the intermediate type.
+int launch_adrt_forward(int n, int dtype, const void* x, void* b, void* work, int S, hipStream_t st);
+int launch_adrt_transpose(int n, int dtype, const void* b, void* x, void* work, int S, hipStream_t st);
+size_t adrt_work_bytes(int n, int dtype, int S, int transpose);
+
+}  // namespace adrt

@@ -1,0 +1,420 @@
+// C ABI, part 2: device-resident inverse plans (L_p and hybrid), their operators and the whole apply
+// (fastops.py:175-338 on the device), hipGraph capture, and the forward / transpose ADRT.
+#include <cmath>
+#include <vector>
+
+#include "../../include/adrt_hip.h"
+#include "passes.hpp"
+#include "rows.hpp"
+
+namespace adrt {
+
+struct Plan {
+    adrt_plan_desc d{};
+    int N = 0, L = 0;
+    double tau = 0;
+    // row parameters
+    c128* z = nullptr;
+    int* deg = nullptr;
+    double* a = nullptr;
+    double* c = nullptr;
+    int* dslot = nullptr;
+    c128* dFT = nullptr;
+    void* tw = nullptr;  // FFT twiddles exp(-2 pi i m / L), m < N, in the row precision
+    // scratch for max_batch sinograms
+    void* Y = nullptr;   // (S, 2, L, N) complex rows
+    void* Sq = nullptr;  // (S, 4, N, N) real quadrant strips
+    double* y = nullptr;
+    double* xs = nullptr;
+    double* res = nullptr;
+    double* r = nullptr;
+    double* g = nullptr;
+    RowOp ops_back[MAX_OPS];
+    int nback = 0;
+    RowOp ops_op[MAX_OPS];
+    int nop = 0;
+};
+
+namespace {
+
+size_t row_elem(int prec) { return prec == PREC_F64 ? sizeof(c128) : sizeof(c64); }
+size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(float); }
+
+void free_plan(Plan* p) {
+    if (!p) return;
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->tw, p->Y, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+int rows_pass(Plan* p, const RowOp* ops, int nops, int S, hipStream_t st) {
+    RowsArgs A{};
+    A.Y = p->Y;
+    A.Z = p->Y;
+    A.rows = (long long)S * 2 * p->L;
+    A.in_stride = A.out_stride = p->N;
+    A.n = p->d.n;
+    A.N = p->N;
+    A.period = p->L;
+    A.z = p->z;
+    A.deg = p->deg;
+    A.a = p->a;
+    A.c = p->c;
+    A.tau = p->tau;
+    A.dslot = p->dslot;
+    A.dFT = p->dFT;
+    A.nops = nops;
+    for (int i = 0; i < nops; ++i) A.ops[i] = ops[i];
+    return launch_rows(A, p->d.precision, st);
+}
+
+// One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
+// mode 0: src is an image batch, mode 1: a sinogram batch.  If resid_y != null, out = resid_y - pass(src).
+int packed_pass(Plan* p, int mode, const double* src, const RowOp* ops, int nops, const double* resid_y, double* out,
+                int S, hipStream_t st) {
+    const int n = p->d.n, prec = p->d.precision;
+    ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, p->tw, st));
+    ADRT_TRY(rows_pass(p, ops, nops, S, st));
+    ADRT_TRY(launch_ifft_unpack(n, p->Y, p->Sq, S, prec, p->tw, st));
+    return launch_combine(n, p->Sq, prec, resid_y, out, S, st);
+}
+
+// P (the outer preconditioner): g1_inv by fixed-degree Chebyshev on [4, 16], lorr, or the identity.
+int precond(Plan* p, const double* v, double* out, int S, hipStream_t st) {
+    const int n = p->d.n;
+    const long long count = (long long)S * p->N * p->N;
+    if (p->d.kind == ADRT_KIND_LP || p->d.precond == ADRT_PRECOND_NONE) {
+        if (out != v) ADRT_HIP(hipMemcpyAsync(out, v, sizeof(double) * count, hipMemcpyDeviceToDevice, st));
+        return OK;
+    }
+    if (p->d.precond == ADRT_PRECOND_LORR) return launch_lorr(n, v, out, S, st);
+    const int deg = p->d.g1_deg;
+    const double lo = 4.0, hi = 16.0;
+    const double c = (hi - lo) / (hi + lo), a = 2.0 / (hi + lo);
+    // u_j lands in buffer (j % 2) with buffer (deg - 1) % 2 == out
+    double* buf[2];
+    buf[(deg - 1) % 2] = out;
+    buf[deg % 2] = p->g;
+    ADRT_TRY(launch_scale(v, a, buf[0], count, st));
+    double om = 1.0;
+    for (int k = 1; k < deg; ++k) {
+        om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
+        const double* xk = buf[(k - 1) % 2];
+        const double* xp = (k == 1) ? nullptr : buf[k % 2];
+        ADRT_TRY(launch_g1_step(n, v, xk, xp, a, om, om - 1.0, buf[k % 2], S, st));
+    }
+    return OK;
+}
+
+int check_batch(Plan* p, int S) {
+    ADRT_REQUIRE(p, "null plan");
+    ADRT_REQUIRE(S >= 0 && S <= p->d.max_batch, "batch larger than the plan's max_batch");
+    return OK;
+}
+
+}  // namespace
+
+}  // namespace adrt
+
+using namespace adrt;
+
+extern "C" {
+
+unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
+    if (!d || d->n < 1 || d->n > 11) return 0;
+    const size_t N = (size_t)1 << d->n, L = 2 * N, S = d->max_batch > 0 ? d->max_batch : 1;
+    size_t b = 0;
+    b += L * (sizeof(c128) + sizeof(int) + 2 * sizeof(double) + sizeof(int));
+    b += (size_t)d->n_dense * N * N * sizeof(c128);
+    b += twiddle_bytes(d->n, d->precision);
+    b += S * 2 * L * N * row_elem(d->precision);
+    b += S * 4 * N * N * strip_elem(d->precision);
+    b += 5 * S * N * N * sizeof(double);
+    return b;
+}
+
+int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
+    ADRT_REQUIRE(out && d, "plan_create: null pointer");
+    *out = nullptr;
+    ADRT_REQUIRE(d->kind == ADRT_KIND_LP || d->kind == ADRT_KIND_HYBRID, "plan_create: kind must be LP or HYBRID");
+    ADRT_REQUIRE(d->n >= 1 && d->n <= 11, "plan_create: need 1 <= n <= 11");
+    ADRT_REQUIRE(d->precision == PREC_F64 || d->precision == PREC_F32, "plan_create: precision must be 0 or 1");
+    ADRT_REQUIRE(d->precond >= 0 && d->precond <= 2, "plan_create: precond must be 0, 1 or 2");
+    ADRT_REQUIRE(d->g1_deg >= 1, "plan_create: g1_deg must be >= 1");
+    ADRT_REQUIRE(d->outer_deg >= 1, "plan_create: outer_deg must be >= 1");
+    ADRT_REQUIRE(d->hi > 0 && d->lo > -d->hi, "plan_create: need hi > 0 and hi + lo > 0");
+    ADRT_REQUIRE(d->max_batch >= 1, "plan_create: max_batch must be >= 1");
+    ADRT_REQUIRE(d->z && d->deg && d->a && d->c, "plan_create: null row parameter array");
+    ADRT_REQUIRE(d->n_dense >= 0 && d->n_dense <= (2 << d->n), "plan_create: bad n_dense");
+    ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && d->dense_F), "plan_create: dense rows need dense_k and dense_F");
+    auto* p = new Plan();
+    p->d = *d;
+    p->d.lam_max = nullptr;
+    p->d.z = nullptr;
+    p->d.deg = nullptr;
+    p->d.a = p->d.c = nullptr;
+    p->d.dense_k = nullptr;
+    p->d.dense_F = nullptr;
+    const int n = d->n, N = 1 << n, L = 2 * N;
+    p->N = N;
+    p->L = L;
+    p->tau = d->tau;
+    auto fail = [&](int st) {
+        free_plan(p);
+        return st;
+    };
+    auto alloc = [&](void** ptr, size_t bytes) -> int { return adrt_malloc(ptr, bytes); };
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("plan_create: upload failed");
+            return E_HIP;
+        }
+        return OK;
+    };
+    int st;
+    if ((st = alloc((void**)&p->z, sizeof(c128) * L)) || (st = up(p->z, d->z, sizeof(c128) * L))) return fail(st);
+    if ((st = alloc((void**)&p->deg, sizeof(int) * L)) || (st = up(p->deg, d->deg, sizeof(int) * L))) return fail(st);
+    if ((st = alloc((void**)&p->a, sizeof(double) * L)) || (st = up(p->a, d->a, sizeof(double) * L))) return fail(st);
+    if ((st = alloc((void**)&p->c, sizeof(double) * L)) || (st = up(p->c, d->c, sizeof(double) * L))) return fail(st);
+    if (d->n_dense > 0) {
+        std::vector<int> slot(L, -1);
+        for (int i = 0; i < d->n_dense; ++i) {
+            if (d->dense_k[i] < 0 || d->dense_k[i] >= L) {
+                set_error("plan_create: dense_k out of range");
+                return fail(E_INVALID);
+            }
+            slot[d->dense_k[i]] = i;
+        }
+        std::vector<c128> ft((size_t)d->n_dense * N * N);
+        const c128* F = (const c128*)d->dense_F;
+        for (int s = 0; s < d->n_dense; ++s)
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j)
+                    ft[(size_t)s * N * N + (size_t)j * N + i] = F[(size_t)s * N * N + (size_t)i * N + j];
+        if ((st = alloc((void**)&p->dslot, sizeof(int) * L)) || (st = up(p->dslot, slot.data(), sizeof(int) * L)))
+            return fail(st);
+        if ((st = alloc((void**)&p->dFT, sizeof(c128) * ft.size())) ||
+            (st = up(p->dFT, ft.data(), sizeof(c128) * ft.size())))
+            return fail(st);
+    }
+    // FFT twiddles exp(-2 pi i m / L) for m < N, from the exact angle in double
+    {
+        std::vector<c128> t64(N);
+        std::vector<c64> t32(N);
+        for (int m = 0; m < N; ++m) {
+            const double ang = -2.0 * M_PI * (double)m / (double)L;
+            t64[m] = mk(std::cos(ang), std::sin(ang));
+            t32[m] = mk((float)t64[m].x, (float)t64[m].y);
+        }
+        const size_t tb = twiddle_bytes(n, d->precision);
+        if ((st = alloc(&p->tw, tb)) || (st = up(p->tw, d->precision == PREC_F64 ? (const void*)t64.data()
+                                                                                 : (const void*)t32.data(), tb)))
+            return fail(st);
+    }
+    const size_t S = d->max_batch, NN = (size_t)N * N;
+    if ((st = alloc(&p->Y, S * 2 * L * N * row_elem(d->precision)))) return fail(st);
+    if ((st = alloc(&p->Sq, S * 4 * NN * strip_elem(d->precision)))) return fail(st);
+    if ((st = alloc((void**)&p->y, S * NN * sizeof(double)))) return fail(st);
+    if ((st = alloc((void**)&p->xs, S * NN * sizeof(double)))) return fail(st);
+    if ((st = alloc((void**)&p->res, S * NN * sizeof(double)))) return fail(st);
+    if ((st = alloc((void**)&p->r, S * NN * sizeof(double)))) return fail(st);
+    if ((st = alloc((void**)&p->g, S * NN * sizeof(double)))) return fail(st);
+    // op lists (fastops.py:281, 285, 330, 335)
+    if (d->kind == ADRT_KIND_HYBRID) {
+        const RowOp back[3] = {{OP_BT, 2, n}, {OP_P, 2, n}, {OP_BT, 1, 1}};
+        const RowOp gh[5] = {{OP_B, 1, 1}, {OP_P, 2, n}, {OP_B, 2, n}, {OP_BT, 2, n}, {OP_BT, 1, 1}};
+        if (n < 2) {
+            set_error("plan_create: the hybrid needs n >= 2");
+            return fail(E_INVALID);
+        }
+        p->nback = 3;
+        p->nop = 5;
+        for (int i = 0; i < 3; ++i) p->ops_back[i] = back[i];
+        for (int i = 0; i < 5; ++i) p->ops_op[i] = gh[i];
+    } else {
+        const RowOp back[2] = {{OP_BT, 1, n}, {OP_P, 1, n}};
+        const RowOp fr[3] = {{OP_P, 1, n}, {OP_B, 1, n}, {OP_BT, 1, n}};
+        p->nback = 2;
+        p->nop = 3;
+        for (int i = 0; i < 2; ++i) p->ops_back[i] = back[i];
+        for (int i = 0; i < 3; ++i) p->ops_op[i] = fr[i];
+    }
+    *out = (adrt_plan*)p;
+    return OK;
+}
+
+int adrt_plan_destroy(adrt_plan* plan) {
+    free_plan((Plan*)plan);
+    return OK;
+}
+
+int adrt_plan_set_outer(adrt_plan* plan, double lo, double hi, int outer_deg) {
+    auto* p = (Plan*)plan;
+    ADRT_REQUIRE(p, "plan_set_outer: null plan");
+    ADRT_REQUIRE(hi > 0 && lo > -hi, "plan_set_outer: need hi > 0 and hi + lo > 0");
+    ADRT_REQUIRE(outer_deg >= 1, "plan_set_outer: outer_deg must be >= 1");
+    p->d.lo = lo;
+    p->d.hi = hi;
+    p->d.outer_deg = outer_deg;
+    return OK;
+}
+
+int adrt_plan_backproject(adrt_plan* plan, const double* d_b, double* d_y, int S, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    if (S == 0) return OK;
+    ADRT_REQUIRE(d_b && d_y, "plan_backproject: null buffer");
+    return packed_pass(p, 1, d_b, p->ops_back, p->nback, nullptr, d_y, S, (hipStream_t)stream);
+}
+
+int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    if (S == 0) return OK;
+    ADRT_REQUIRE(d_x && d_y, "plan_operator: null buffer");
+    return packed_pass(p, 0, d_x, p->ops_op, p->nop, nullptr, d_y, S, (hipStream_t)stream);
+}
+
+int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    if (S == 0) return OK;
+    ADRT_REQUIRE(d_x && d_y && d_x != d_y, "plan_precond: null or aliased buffer");
+    return precond(p, d_x, d_y, S, (hipStream_t)stream);
+}
+
+// x = outer Chebyshev of degree D on P G with right-hand side y = backproject(b) (fastops.py:247-258, 287-288, 337-338)
+int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    if (S == 0) return OK;
+    ADRT_REQUIRE(d_b && d_x, "plan_apply: null buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const long long count = (long long)S * p->N * p->N;
+    const double c = (p->d.hi - p->d.lo) / (p->d.hi + p->d.lo), a = 2.0 / (p->d.hi + p->d.lo);
+    const int D = p->d.outer_deg;
+    ADRT_TRY(packed_pass(p, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st));
+    // iterate x_j lands in buffer j % 2, with buffer (D - 1) % 2 == d_x
+    double* X[2];
+    X[(D - 1) % 2] = d_x;
+    X[D % 2] = p->xs;
+    const bool has_p = !(p->d.kind == ADRT_KIND_LP || p->d.precond == ADRT_PRECOND_NONE);
+    const double* py = p->y;
+    if (has_p) {
+        ADRT_TRY(precond(p, p->y, p->r, S, st));
+        py = p->r;
+    }
+    ADRT_TRY(launch_scale(py, a, X[0], count, st));
+    double om = 1.0;
+    for (int k = 1; k < D; ++k) {
+        om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
+        const double* xk = X[(k - 1) % 2];
+        const double* xp = (k == 1) ? nullptr : X[k % 2];
+        ADRT_TRY(packed_pass(p, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st));
+        const double* rr = p->res;
+        if (has_p) {
+            ADRT_TRY(precond(p, p->res, p->r, S, st));
+            rr = p->r;
+        }
+        ADRT_TRY(launch_cheb_update(xk, xp, rr, a, om, om - 1.0, X[k % 2], count, st));
+    }
+    return OK;
+}
+
+// ---------------------------------------------------------------- graphs
+struct adrt_graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+};
+
+int adrt_graph_begin(void* stream) {
+    ADRT_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+    return OK;
+}
+
+int adrt_graph_end(void* stream, adrt_graph** out) {
+    ADRT_REQUIRE(out, "graph_end: null out pointer");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    ADRT_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(err));
+        return E_HIP;
+    }
+    auto* G = new adrt_graph();
+    G->g = g;
+    G->e = e;
+    *out = G;
+    return OK;
+}
+
+int adrt_graph_launch(adrt_graph* g, void* stream) {
+    ADRT_REQUIRE(g && g->e, "graph_launch: null graph");
+    ADRT_HIP(hipGraphLaunch(g->e, (hipStream_t)stream));
+    return OK;
+}
+
+int adrt_graph_destroy(adrt_graph* g) {
+    if (!g) return OK;
+    if (g->e) (void)hipGraphExecDestroy(g->e);
+    if (g->g) (void)hipGraphDestroy(g->g);
+    delete g;
+    return OK;
+}
+
+int adrt_stream_wait(void* waiting, void* signal) {
+    hipEvent_t ev;
+    ADRT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e1 = hipEventRecord(ev, (hipStream_t)signal);
+    hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent((hipStream_t)waiting, ev, 0) : e1;
+    (void)hipEventDestroy(ev);
+    if (e2 != hipSuccess) {
+        set_error(std::string("stream_wait failed: ") + hipGetErrorString(e2));
+        return E_HIP;
+    }
+    return OK;
+}
+
+// ---------------------------------------------------------------- timing events (for in-process measurement)
+int adrt_event_create(void** ev) {
+    ADRT_REQUIRE(ev, "event_create: null pointer");
+    hipEvent_t e;
+    ADRT_HIP(hipEventCreate(&e));
+    *ev = (void*)e;
+    return OK;
+}
+
+int adrt_event_destroy(void* ev) {
+    if (ev) ADRT_HIP(hipEventDestroy((hipEvent_t)ev));
+    return OK;
+}
+
+int adrt_event_record(void* ev, void* stream) {
+    ADRT_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return OK;
+}
+
+int adrt_event_elapsed_ms(void* start, void* stop, float* ms) {
+    ADRT_REQUIRE(ms, "event_elapsed: null pointer");
+    ADRT_HIP(hipEventSynchronize((hipEvent_t)stop));
+    ADRT_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return OK;
+}
+
+// ---------------------------------------------------------------- forward / transpose ADRT
+unsigned long long adrt_work_bytes_fwd(int n, int dtype, int S) { return adrt_work_bytes(n, dtype, S, 0); }
+unsigned long long adrt_work_bytes_T(int n, int dtype, int S) { return adrt_work_bytes(n, dtype, S, 1); }
+
+int adrt_forward(int n, int dtype, const void* d_x, void* d_b, void* d_work, int S, void* stream) {
+    return launch_adrt_forward(n, dtype, d_x, d_b, d_work, S, (hipStream_t)stream);
+}
+
+int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, int S, void* stream) {
+    return launch_adrt_transpose(n, dtype, d_b, d_x, d_work, S, (hipStream_t)stream);
+}
+
+}  // extern "C"

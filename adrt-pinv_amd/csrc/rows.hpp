@@ -1,0 +1,41 @@
+// Fused per-frequency row kernel: butterfly levels, adjoints and the Chebyshev filter of one row, in LDS.
+//
+// Index scheme.  The reference writes level m out of place (cpp/oblique.cpp:22-50): outputs l2w+2t+b from inputs
+// l2w+t and l2w+w+t.  Here every level is applied IN PLACE on the pair of physical slots (p0, p0 + w) that differ
+// in bit m-1.  After forward levels lo..hi the physical slot p holds logical index Lambda(p) = rotl_hi(...rotl_lo(p))
+// (rotl_k rotates the low k bits left by one); for lo <= 2 this is the bit reversal of the low hi bits.  The twiddle
+// of the pair at physical p0 on level m is t = low (m-1) bits of Lambda_{lo..m-1}(p0).  The adjoint runs the same
+// pairs in reverse level order and returns the natural order, so a B..BT sandwich (the Gram M = B^H B inside the
+// Chebyshev filter) never permutes data.
+#pragma once
+#include "common.hpp"
+
+namespace adrt {
+
+struct RowsArgs {
+    const void* Y;          // input rows, complex<Tg>, row r at Y + r * in_stride
+    void* Z;                // output rows (may alias Y)
+    long long rows;         // number of rows
+    long long in_stride;    // complex elements between rows
+    long long out_stride;
+    int n, N;
+    int period;             // parameter index of row r is r % period
+    const c128* z;          // (period) z = exp(i w) of each parameter row
+    const int* deg;         // (period) Chebyshev degree (P ops)
+    const double* a;        // (period) Chebyshev constants
+    const double* c;
+    double tau;
+    const int* dslot;       // (period) dense-table slot of the row or -1; null when there are no dense rows
+    const c128* dFT;        // dense tables, slot s at dFT + s*N*N, stored transposed: dFT[j*N + i] = F[i][j]
+    int nops;
+    RowOp ops[MAX_OPS];
+};
+
+// Host launcher (rows.hip).  prec selects the compute type of the butterflies and Chebyshev steps; the global row
+// type is complex128 for PREC_F64 and complex64 for PREC_F32.
+int launch_rows(const RowsArgs& args, int prec, hipStream_t stream);
+
+// Validates an op list against n; returns OK or E_INVALID with the error set.
+int check_ops(int n, const RowOp* ops, int nops);
+
+}  // namespace adrt

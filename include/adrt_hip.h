@@ -1,0 +1,146 @@
+/* C ABI of libadrt_hip.so: the MI355X engine for the ADRT oblique inverses of dsrim/adrt-pinv.
+ *
+ * Two layers.
+ *  1. Reference-compatible row kernels on HOST buffers, same names and argument meaning as the C++ engine that
+ *     fastcpp.py binds with ctypes (cpp/oblique.cpp:53-139; argtypes at fastcpp.py:25-31).  They copy in, run on the
+ *     GPU and copy out.  `nthreads` is accepted and ignored.  Like the reference they return void; on a bad argument or
+ *     a HIP failure they print one line to stderr and set adrt_last_error().
+ *  2. A device-resident API (device pointers, explicit HIP streams, int status codes: 0 ok, -1 invalid argument,
+ *     -2 HIP error, -3 out of memory, -4 bad state) for the row engine, the whole L_p / hybrid apply, hipGraph capture
+ *     and the forward / transpose ADRT.
+ *
+ * Complex numbers are interleaved (re, im) doubles, the layout of numpy complex128 and std::complex<double>.
+ * A stream argument of NULL means the default stream.  No call keeps a pointer it was given beyond the call, except
+ * the plans, which copy their parameters to the device at creation.
+ */
+#ifndef ADRT_HIP_H
+#define ADRT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double re, im;
+} adrt_c128;
+
+/* ------------------------------------------------------------------ layer 1: reference-compatible (host buffers) */
+
+/* X: K rows of length N = 2^n, in place.  transpose = 0: B_{m_hi} ... B_{m_lo}; 1: the adjoint.
+ * Replaces bfly_rows of cpp/oblique.cpp:56 (bound at fastcpp.py:25). */
+void bfly_rows(int n, int K, const adrt_c128* z, adrt_c128* X, int m_lo, int m_hi, int transpose, int nthreads);
+
+/* Z[k] = p_k(M) Y[k] with deg[k] Chebyshev steps, constants a[k], c[k], M = B^H B over levels m_lo..m_hi;
+ * *steps += sum_k max(deg[k]-1, 0) * 2 (m_hi - m_lo + 1) when steps != NULL.
+ * Replaces cheb_rows of cpp/oblique.cpp:74-75 (bound at fastcpp.py:27). */
+void cheb_rows(int n, int K, const adrt_c128* z, const int* deg, const double* a, const double* c, double tau,
+               const adrt_c128* Y, adrt_c128* Z, int m_lo, int m_hi, int nthreads, long long* steps);
+
+/* Certified upper bounds Lambda_k >= lambda_max(M_k) by the row-sum recursion over levels m_lo..n.
+ * Replaces cert_rowsum of cpp/oblique.cpp:108 (bound at fastcpp.py:29). */
+void cert_rowsum(int n, int K, const double* omega, int m_lo, double* out, int nthreads);
+
+/* Parallel width of the engine: the compute-unit count of the current GPU.
+ * Replaces omp_max_threads of cpp/oblique.cpp:132 (bound at fastcpp.py:31). */
+int omp_max_threads(void);
+
+/* Status-returning variants of the three kernels above (same arguments, no nthreads). */
+int adrt_bfly_rows(int n, int K, const adrt_c128* z, adrt_c128* X, int m_lo, int m_hi, int transpose);
+int adrt_cheb_rows(int n, int K, const adrt_c128* z, const int* deg, const double* a, const double* c, double tau,
+                   const adrt_c128* Y, adrt_c128* Z, int m_lo, int m_hi, long long* steps);
+int adrt_cert_rowsum(int n, int K, const double* omega, int m_lo, double* out);
+
+/* ------------------------------------------------------------------ layer 2: device-resident API */
+
+const char* adrt_last_error(void); /* message of the last failure on this thread */
+int adrt_device_count(void);
+int adrt_malloc(void** ptr, unsigned long long bytes);
+int adrt_free(void* ptr);
+/* kind: 0 host->host, 1 host->device, 2 device->host (synchronous), 3 device->device (stream-ordered) */
+int adrt_memcpy(void* dst, const void* src, unsigned long long bytes, int kind, void* stream);
+int adrt_memset(void* dst, int value, unsigned long long bytes, void* stream);
+int adrt_stream_create(void** stream);
+int adrt_stream_destroy(void* stream);
+int adrt_stream_sync(void* stream);
+int adrt_stream_wait(void* waiting, void* signal); /* `waiting` waits for the work enqueued so far on `signal` */
+int adrt_device_sync(void);
+int adrt_device_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
+int adrt_event_create(void** ev);
+int adrt_event_destroy(void* ev);
+int adrt_event_record(void* ev, void* stream);
+int adrt_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* Row engine with device-resident parameters: the plug-in surface of fastops.RowEngine (fastops.py:20-167).
+ * Row r of a run uses parameter row r % L.  Rows whose parameter row is listed in dense_k use the exact filter
+ * dense_F[i] (N x N, row-major, complex128) in P ops.  precision 0: complex128 rows; 1: complex64 rows.
+ * ops: nops triples (kind, lo, hi), kind 0 = B, 1 = BT, 2 = P, applied left to right (fastops.py:156-167). */
+typedef struct adrt_rows_plan adrt_rows_plan;
+int adrt_rows_plan_create(adrt_rows_plan** out, int n, int L, const adrt_c128* z, const int* deg, const double* a,
+                          const double* c, double tau, int n_dense, const int* dense_k, const adrt_c128* dense_F,
+                          int precision);
+int adrt_rows_plan_destroy(adrt_rows_plan* plan);
+int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows, const int* ops, int nops,
+                  void* stream);
+
+/* Inverse plans: fastops.FastLp / FastHybrid (fastops.py:261-338) with the setup scalars given.
+ * All image / sinogram buffers are float64 on the device:
+ *   b: (S, 4, 2N-1, N) sinograms, x / y: (S, N, N) images. */
+enum { ADRT_KIND_LP = 0, ADRT_KIND_HYBRID = 1 };
+enum { ADRT_PRECOND_NONE = 0, ADRT_PRECOND_G1 = 1, ADRT_PRECOND_LORR = 2 };
+enum { ADRT_PREC_F64 = 0, ADRT_PREC_F32 = 1 };
+typedef struct {
+    int kind;           /* ADRT_KIND_* */
+    int n;              /* N = 2^n, 1 <= n <= 11 (hybrid: n >= 2) */
+    int precision;      /* ADRT_PREC_F64: every op in double.  ADRT_PREC_F32: offset FFTs, butterflies and Chebyshev
+                           steps in float; dense table, preconditioner and outer iteration in double */
+    int precond;        /* hybrid only: ADRT_PRECOND_* */
+    int g1_deg;         /* Chebyshev degree of G_1^{-1} (fastops default 14) */
+    int outer_deg;      /* D, the outer Chebyshev degree */
+    double lo, hi;      /* outer interval */
+    double tau;         /* Tikhonov shift tau_fac * N */
+    const double* lam_max; /* (L) symmetrised bounds; informational, may be NULL */
+    const adrt_c128* z; /* (L) exp(i w_k), w = 2 pi fftfreq(2N) */
+    const int* deg;     /* (L) inner degrees */
+    const double* a;    /* (L) Chebyshev constants (fastspec.cheb_coeffs) */
+    const double* c;    /* (L) */
+    int n_dense;        /* frequencies with the exact filter */
+    const int* dense_k; /* (n_dense) their indices in [0, L) */
+    const adrt_c128* dense_F; /* (n_dense, N, N) row-major */
+    int max_batch;      /* sinograms per call; scratch is sized for it */
+} adrt_plan_desc;
+
+typedef struct adrt_plan adrt_plan;
+unsigned long long adrt_plan_bytes(const adrt_plan_desc* desc); /* device bytes a plan holds (admission control) */
+int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* desc);
+int adrt_plan_destroy(adrt_plan* plan);
+/* replaces the outer interval and degree (set after the setup's eigenvalue estimate) */
+int adrt_plan_set_outer(adrt_plan* plan, double lo, double hi, int outer_deg);
+/* y = backproject(b) (fastops.py:283-285 / 333-335) */
+int adrt_plan_backproject(adrt_plan* plan, const double* d_b, double* d_y, int S, void* stream);
+/* y = G x: frame_window (L_p, fastops.py:277-281) or G_h (hybrid, fastops.py:325-331) */
+int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
+/* y = P x: g1_inv / lorr_inv (hybrid) or the identity (fasthybrid.py:101-130) */
+int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
+/* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
+int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
+
+/* hipGraph capture of everything enqueued on `stream` (and streams joined through adrt_stream_wait). */
+typedef struct adrt_graph adrt_graph;
+int adrt_graph_begin(void* stream);
+int adrt_graph_end(void* stream, adrt_graph** out);
+int adrt_graph_launch(adrt_graph* graph, void* stream);
+int adrt_graph_destroy(adrt_graph* graph);
+
+/* Forward ADRT R: (S, N, N) -> (S, 4, 2N-1, N) and its transpose (adrtcore.py:122-137).
+ * dtype 0: int32 in and out (forward), int32 in / int64 out (transpose); 1: float32; 2: float64.
+ * d_work: adrt_work_bytes_fwd / adrt_work_bytes_T bytes of device scratch. */
+unsigned long long adrt_work_bytes_fwd(int n, int dtype, int S);
+unsigned long long adrt_work_bytes_T(int n, int dtype, int S);
+int adrt_forward(int n, int dtype, const void* d_x, void* d_b, void* d_work, int S, void* stream);
+int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, int S, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADRT_HIP_H */

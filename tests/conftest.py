@@ -1,0 +1,42 @@
+"""Test configuration: the ``gpu`` marker, import helpers for the package (its directory name has a hyphen) and
+the golden fixtures."""
+
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+def pkg(sub=None):
+    name = "adrt-pinv_amd" + (f".{sub}" if sub else "")
+    return importlib.import_module(name)
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def relmax(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """The loaded HIP library; fails (does not skip) when no GPU is visible, so a GPU run cannot pass vacuously."""
+    L = pkg("lib")
+    lib = L.load()
+    n = lib.adrt_device_count()
+    assert n >= 1, "no GPU visible to the HIP runtime"
+    return L

@@ -1,0 +1,96 @@
+"""Row kernels on the GPU against the oracle and the reference's RowEngine fixtures (fastops.py:156-167)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import golden, pkg, relmax
+from oracle import rows as orw
+
+pytestmark = pytest.mark.gpu
+
+OPS = lambda n: {
+    "B_1n": [("B", 1, n)], "BT_1n": [("BT", 1, n)], "B_2n": [("B", 2, n)], "BT_2n": [("BT", 2, n)],
+    "B_11": [("B", 1, 1)], "BT_11": [("BT", 1, 1)], "P_1n": [("P", 1, n)], "P_2n": [("P", 2, n)],
+    "gh": [("B", 1, 1), ("P", 2, n), ("B", 2, n), ("BT", 2, n), ("BT", 1, 1)],
+    "hyb_back": [("BT", 2, n), ("P", 2, n), ("BT", 1, 1)],
+    "lp_frame": [("P", 1, n), ("B", 1, n), ("BT", 1, n)],
+    "lp_back": [("BT", 1, n), ("P", 1, n)],
+}
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 8, 9, 11])
+@pytest.mark.parametrize("levels", ["full", "upper", "one", "mid"])
+@pytest.mark.parametrize("transpose", [0, 1])
+def test_bfly_rows_vs_oracle(gpu, n, levels, transpose):
+    lo, hi = {"full": (1, n), "upper": (min(2, n), n), "one": (1, 1), "mid": (min(3, n), max(min(3, n), n - 1))}[levels]
+    N = 1 << n
+    K = 64 if n < 11 else 8
+    rng = np.random.default_rng(n * 10 + transpose)
+    z = np.exp(1j * rng.uniform(-np.pi, np.pi, K))
+    X = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    lib = gpu.load()
+    Xg = X.copy()
+    lib.bfly_rows(n, K, gpu.ptr(z), gpu.ptr(Xg), lo, hi, transpose, 1)
+    ref = (orw.bflyT if transpose else orw.bfly)(n, z, X, lo, hi)
+    assert relmax(Xg, ref) < 1e-13
+
+
+@pytest.mark.parametrize("n,lo", [(4, 1), (6, 2), (8, 2), (10, 1)])
+def test_cheb_rows_vs_oracle(gpu, n, lo):
+    N = 1 << n
+    K = 96
+    rng = np.random.default_rng(n)
+    z = np.exp(1j * 2 * np.pi * rng.integers(0, 2 * N, K) / (2 * N))
+    deg = rng.integers(1, 40, K).astype(np.int32)
+    lam = np.abs(rng.standard_normal(K)) * N * N * 0.3 + 1.0
+    tau = 0.2 * N
+    c, a = orw.cheb_coeffs(lam, tau)
+    Y = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    Z = np.empty_like(Y)
+    steps = ctypes.c_longlong(5)
+    gpu.load().cheb_rows(n, K, gpu.ptr(z), gpu.ptr(deg), gpu.ptr(a), gpu.ptr(c), tau, gpu.ptr(Y), gpu.ptr(Z), lo, n, 1,
+                         ctypes.byref(steps))
+    ref, s = orw.cheb_filter(n, z, Y, deg, a, c, tau, lo, n)
+    assert relmax(Z, ref) < 1e-12
+    assert steps.value == 5 + s
+
+
+@pytest.mark.parametrize("n", [3, 6, 10])
+@pytest.mark.parametrize("lo", [1, 2])
+def test_cert_rowsum_vs_oracle(gpu, n, lo):
+    oms = 2 * np.pi * np.fft.fftfreq(2 << n)
+    out = np.empty(oms.size)
+    gpu.load().cert_rowsum(n, oms.size, gpu.ptr(oms), lo, gpu.ptr(out), 1)
+    ref = np.array([orw.cert_rowsum(n, float(o), lo) for o in oms])
+    assert relmax(out, ref) < 1e-13
+
+
+@pytest.mark.parametrize("fixture", ["rows_n4_cap-1", "rows_n4_cap8", "rows_n5_cap8"])
+@pytest.mark.parametrize("p", [1, 2])
+def test_engine_vs_reference_fixture(gpu, fixture, p):
+    """HipEngine.run on the reference RowEngine's random rows, every op list (fixtures from fastops.RowEngine)."""
+    d = golden(fixture)
+    n = int(d["n"])
+    cap = int(d["dense_cap"])
+    E = pkg("engine")
+    eng = E.HipEngine(n, d["z"], d["deg"], d["lam_max"], float(d["tau"]), copies=2,
+                      dense_cap=None if cap < 0 else cap, p_levels=(p, None))
+    assert eng.n_dense == int(d[f"n_dense_p{p}"])
+    for name, ops in OPS(n).items():
+        eng.count = [0]
+        Z = eng.run(d["Y"], ops)
+        assert relmax(Z, d[f"{name}_p{p}"]) < 1e-12, name
+        assert eng.count[0] == int(d[f"{name}_p{p}_count"]), name
+
+
+def test_engine_fp32_close(gpu):
+    d = golden("rows_n5_cap8")
+    n = int(d["n"])
+    E = pkg("engine")
+    eng = E.HipEngine(n, d["z"], d["deg"], d["lam_max"], float(d["tau"]), dense_cap=None, precision="fp32")
+    Z = eng.run(d["Y"], OPS(n)["gh"])
+    eng64 = E.HipEngine(n, d["z"], d["deg"], d["lam_max"], float(d["tau"]), dense_cap=None)
+    Z64 = eng64.run(d["Y"], OPS(n)["gh"])
+    assert relmax(Z, Z64) < 1e-4
