@@ -23,6 +23,7 @@ struct Plan {
     void* tw = nullptr;  // FFT twiddles exp(-2 pi i m / L), m < N, in the row precision
     // scratch for max_batch sinograms
     void* Y = nullptr;   // (S, 2, L, N) complex rows
+    void* Y2 = nullptr;  // second row buffer, allocated on first use by adrt_plan_rows_pass
     void* Sq = nullptr;  // (S, 4, N, N) real quadrant strips
     double* y = nullptr;
     double* xs = nullptr;
@@ -42,16 +43,16 @@ size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(
 
 void free_plan(Plan* p) {
     if (!p) return;
-    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->tw, p->Y, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->tw, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
 }
 
-int rows_pass(Plan* p, const RowOp* ops, int nops, int S, hipStream_t st) {
+int rows_pass(Plan* p, const RowOp* ops, int nops, int S, hipStream_t st, void* out = nullptr) {
     RowsArgs A{};
     A.Y = p->Y;
-    A.Z = p->Y;
+    A.Z = out ? out : p->Y;
     A.rows = (long long)S * 2 * p->L;
     A.in_stride = A.out_stride = p->N;
     A.n = p->d.n;
@@ -282,6 +283,18 @@ int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, vo
     if (S == 0) return OK;
     ADRT_REQUIRE(d_x && d_y && d_x != d_y, "plan_precond: null or aliased buffer");
     return precond(p, d_x, d_y, S, (hipStream_t)stream);
+}
+
+// The row kernel alone on the plan's current rows (the last packed pass), out of place into a second buffer so that
+// repeated launches see identical input: the dominant kernel of an apply, for timing with events.
+int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    ADRT_REQUIRE(which == 0 || which == 1, "plan_rows_pass: which must be 0 (backproject ops) or 1 (operator ops)");
+    if (S == 0) return OK;
+    if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(p->d.precision)));
+    return which == 0 ? rows_pass(p, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2)
+                      : rows_pass(p, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2);
 }
 
 // x = outer Chebyshev of degree D on P G with right-hand side y = backproject(b) (fastops.py:247-258, 287-288, 337-338)

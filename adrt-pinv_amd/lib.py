@@ -54,6 +54,7 @@ SIGNATURES = {
     "adrt_plan_backproject": (I, [P, P, P, I, P]),
     "adrt_plan_operator": (I, [P, P, P, I, P]),
     "adrt_plan_precond": (I, [P, P, P, I, P]),
+    "adrt_plan_rows_pass": (I, [P, I, I, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),
     "adrt_graph_begin": (I, [P]),
     "adrt_graph_end": (I, [P, ctypes.POINTER(P)]),

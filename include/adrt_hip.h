@@ -121,6 +121,9 @@ int adrt_plan_backproject(adrt_plan* plan, const double* d_b, double* d_y, int S
 int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
 /* y = P x: g1_inv / lorr_inv (hybrid) or the identity (fasthybrid.py:101-130) */
 int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
+/* the row kernel alone on the rows of the plan's last packed pass (which: 0 backproject ops, 1 operator ops), written
+ * to a second buffer: the dominant kernel of an apply, for timing */
+int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream);
 /* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
 
