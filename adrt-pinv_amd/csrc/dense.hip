@@ -1,0 +1,113 @@
+// Dense low-frequency rows as a batched complex GEMM.
+//
+// For a frequency k whose degree exceeds dense_cap, the reference applies the exact Tikhonov filter F_k inside the op
+// list (fastops.py:28-39, 104).  Every op of a packed pass is linear and row-local, so the whole op list of such a row
+// is one N x N matrix W_k (e.g. B1^H M' F B1 for G_h), built once at plan creation.  A pass then multiplies the
+// 2S rows of frequency k (both packed pairs of every sinogram) by W_k: out_r = W_k in_r.  One block owns TR rows of
+// one frequency and all N outputs, so it may write its rows in place; W_k^T is streamed from L2/HBM (coalesced over
+// the outputs) and reused by the block's TR rows, whose inputs are staged through LDS in chunks of TJ columns.
+#include "rows.hpp"
+
+namespace adrt {
+
+namespace {
+
+constexpr int DG_THREADS = 256;
+constexpr int DG_TJ = 32;
+
+template <typename Tg, int PER, int TR>
+__global__ __launch_bounds__(DG_THREADS) void dense_gemm_kernel(int n, int L, int SP, const int* dense_k,
+                                                                const c128* WT, const void* Yin, void* Yout) {
+    __shared__ c128 ys[TR][DG_TJ];
+    const int N = 1 << n;
+    const int slot = blockIdx.y;
+    const int k = dense_k[slot];
+    const int sp0 = blockIdx.x * TR;
+    const c128* W = WT + (size_t)slot * N * N;
+    const cplx<Tg>* Y = reinterpret_cast<const cplx<Tg>*>(Yin);
+    cplx<Tg>* Z = reinterpret_cast<cplx<Tg>*>(Yout);
+    c128 acc[TR][PER];
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+#pragma unroll
+        for (int e = 0; e < PER; ++e) acc[r][e] = mk(0.0, 0.0);
+    for (int j0 = 0; j0 < N; j0 += DG_TJ) {
+        for (int idx = threadIdx.x; idx < TR * DG_TJ; idx += DG_THREADS) {
+            const int r = idx / DG_TJ, jj = idx % DG_TJ;
+            const int sp = sp0 + r;
+            ys[r][jj] = (sp < SP && j0 + jj < N) ? cast<double>(Y[((size_t)sp * L + k) * N + j0 + jj]) : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        const int jn = (N - j0) < DG_TJ ? (N - j0) : DG_TJ;
+        // eight columns of W^T in flight per step (the loop is otherwise bound by load latency)
+        for (int jb = 0; jb < jn; jb += 8) {
+            c128 w[8][PER];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int e = 0; e < PER; ++e) {
+                    const int i = threadIdx.x + DG_THREADS * e;
+                    w[q][e] = (i < N && jb + q < jn) ? W[(size_t)(j0 + jb + q) * N + i] : mk(0.0, 0.0);
+                }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+#pragma unroll
+                for (int r = 0; r < TR; ++r) {
+                    const c128 yv = ys[r][jb + q < DG_TJ ? jb + q : 0];
+#pragma unroll
+                    for (int e = 0; e < PER; ++e) acc[r][e] = add(acc[r][e], mul(w[q][e], yv));
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const int sp = sp0 + r;
+        if (sp >= SP) continue;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int i = threadIdx.x + DG_THREADS * e;
+            if (i < N) Z[((size_t)sp * L + k) * N + i] = cast<Tg>(acc[r][e]);
+        }
+    }
+}
+
+template <typename Tg, int PER, int TR>
+int launch_t(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
+             hipStream_t st) {
+    const unsigned gx = (unsigned)((SP + TR - 1) / TR);
+    hipLaunchKernelGGL((dense_gemm_kernel<Tg, PER, TR>), dim3(gx, n_dense), dim3(DG_THREADS), 0, st, n, L, SP, dense_k,
+                       WT, Y, Z);
+    return launch_check("dense_gemm_kernel");
+}
+
+template <typename Tg>
+int launch_n(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
+             hipStream_t st) {
+    // PER outputs per thread x TR rows per block; few rows (one sinogram) get a narrow TR so no block idles
+    const bool few = SP <= 4;
+    if (n <= 8)
+        return few ? launch_t<Tg, 1, 2>(n, L, SP, n_dense, dense_k, WT, Y, Z, st)
+                   : launch_t<Tg, 1, 16>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    if (n == 9)
+        return few ? launch_t<Tg, 2, 2>(n, L, SP, n_dense, dense_k, WT, Y, Z, st)
+                   : launch_t<Tg, 2, 8>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    if (n == 10)
+        return few ? launch_t<Tg, 4, 2>(n, L, SP, n_dense, dense_k, WT, Y, Z, st)
+                   : launch_t<Tg, 4, 4>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    return launch_t<Tg, 8, 2>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+}
+
+}  // namespace
+
+// Z_r = W_slot Y_r for the SP rows of every dense frequency (rows sp * L + k); Z may alias Y.
+int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
+                      int prec, hipStream_t st) {
+    if (n_dense <= 0 || SP <= 0) return OK;
+    ADRT_REQUIRE(n >= 1 && n <= 11, "dense_gemm: need 1 <= n <= 11");
+    if (prec == PREC_F64) return launch_n<double>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    return launch_n<float>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+}
+
+}  // namespace adrt

@@ -31,6 +31,10 @@ int launch_cheb_update(const double* x, const double* xp, const double* r, doubl
 // means the zero start of the first step.
 int launch_g1_step(int n, const double* y, const double* x, const double* xp, double a, double w1, double w0,
                    double* out, int S, hipStream_t st);
+// g1_inv(y) (deg Chebyshev steps of G_1 on [4, 16]) in one launch, with an epilogue:
+//   mode 0: out = P y;  mode 1: out = a_out * P y;  mode 2: out = x + w1 * (a_out * P y) + w0 * (x - xp) (xp null = 0)
+int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
+                    double w1, double w0, double* out, int S, hipStream_t st);
 // out = (y - Pi y)/4 + Pi y / 12 (fasthybrid.py:101-111)
 int launch_lorr(int n, const double* y, double* out, int S, hipStream_t st);
 

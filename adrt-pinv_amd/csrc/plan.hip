@@ -1,5 +1,6 @@
 // C ABI, part 2: device-resident inverse plans (L_p and hybrid), their operators and the whole apply
 // (fastops.py:175-338 on the device), hipGraph capture, and the forward / transpose ADRT.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -20,7 +21,13 @@ struct Plan {
     double* c = nullptr;
     int* dslot = nullptr;
     c128* dFT = nullptr;
-    void* tw = nullptr;  // FFT twiddles exp(-2 pi i m / L), m < N, in the row precision
+    int* korder = nullptr;  // frequencies by descending degree (block order of the row kernel)
+    int* korder_nd = nullptr;  // the same without the dense frequencies
+    int* dense_k = nullptr;    // dense frequencies (GEMM slots)
+    c128* WopT = nullptr;      // (n_dense, N, N) pass matrices, transposed
+    c128* WbackT = nullptr;
+    void* tw = nullptr;    // FFT twiddles exp(-2 pi i m / L), m < N, double
+    void* tw32 = nullptr;  // the same rounded to float
     // scratch for max_batch sinograms
     void* Y = nullptr;   // (S, 2, L, N) complex rows
     void* Y2 = nullptr;  // second row buffer, allocated on first use by adrt_plan_rows_pass
@@ -43,13 +50,14 @@ size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(
 
 void free_plan(Plan* p) {
     if (!p) return;
-    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->tw, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
 }
 
-int rows_pass(Plan* p, const RowOp* ops, int nops, int S, hipStream_t st, void* out = nullptr) {
+int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t st, void* out = nullptr,
+              const c128* WT = nullptr) {
     RowsArgs A{};
     A.Y = p->Y;
     A.Z = out ? out : p->Y;
@@ -67,17 +75,29 @@ int rows_pass(Plan* p, const RowOp* ops, int nops, int S, hipStream_t st, void* 
     A.dFT = p->dFT;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = ops[i];
-    return launch_rows(A, p->d.precision, st);
+    A.group_by_k = 1;
+    A.SP = 2 * S;
+    A.korder = p->korder;
+    if (WT && p->d.n_dense > 0) {
+        // Chebyshev rows in the row kernel, dense rows as one GEMM with the pass matrix
+        A.korder = p->korder_nd;
+        A.nk = p->L - p->d.n_dense;
+        A.skip_dense = 1;
+        if (A.nk > 0) ADRT_TRY(launch_rows(A, prec, st));
+        return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
+    }
+    return launch_rows(A, prec, st);
 }
 
 // One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
 // mode 0: src is an image batch, mode 1: a sinogram batch.  If resid_y != null, out = resid_y - pass(src).
-int packed_pass(Plan* p, int mode, const double* src, const RowOp* ops, int nops, const double* resid_y, double* out,
-                int S, hipStream_t st) {
-    const int n = p->d.n, prec = p->d.precision;
-    ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, p->tw, st));
-    ADRT_TRY(rows_pass(p, ops, nops, S, st));
-    ADRT_TRY(launch_ifft_unpack(n, p->Y, p->Sq, S, prec, p->tw, st));
+int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops, int nops, const double* resid_y,
+                double* out, int S, hipStream_t st) {
+    const int n = p->d.n;
+    const void* tw = prec == PREC_F64 ? p->tw : p->tw32;
+    ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, tw, st));
+    ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, nullptr, mode == 1 ? p->WbackT : p->WopT));
+    ADRT_TRY(launch_ifft_unpack(n, p->Y, p->Sq, S, prec, tw, st));
     return launch_combine(n, p->Sq, prec, resid_y, out, S, st);
 }
 
@@ -91,6 +111,7 @@ int precond(Plan* p, const double* v, double* out, int S, hipStream_t st) {
     }
     if (p->d.precond == ADRT_PRECOND_LORR) return launch_lorr(n, v, out, S, st);
     const int deg = p->d.g1_deg;
+    if (deg - 1 <= 16) return launch_g1_fused(n, deg, 0, v, nullptr, nullptr, 1.0, 0.0, 0.0, out, S, st);
     const double lo = 4.0, hi = 16.0;
     const double c = (hi - lo) / (hi + lo), a = 2.0 / (hi + lo);
     // u_j lands in buffer (j % 2) with buffer (deg - 1) % 2 == out
@@ -127,10 +148,10 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
     const size_t N = (size_t)1 << d->n, L = 2 * N, S = d->max_batch > 0 ? d->max_batch : 1;
     size_t b = 0;
     b += L * (sizeof(c128) + sizeof(int) + 2 * sizeof(double) + sizeof(int));
-    b += (size_t)d->n_dense * N * N * sizeof(c128);
-    b += twiddle_bytes(d->n, d->precision);
-    b += S * 2 * L * N * row_elem(d->precision);
-    b += S * 4 * N * N * strip_elem(d->precision);
+    b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
+    b += twiddle_bytes(d->n, PREC_F64) + twiddle_bytes(d->n, PREC_F32);
+    b += S * 2 * L * N * row_elem(PREC_F64);
+    b += S * 4 * N * N * strip_elem(PREC_F64);
     b += 5 * S * N * N * sizeof(double);
     return b;
 }
@@ -146,6 +167,7 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     ADRT_REQUIRE(d->outer_deg >= 1, "plan_create: outer_deg must be >= 1");
     ADRT_REQUIRE(d->hi > 0 && d->lo > -d->hi, "plan_create: need hi > 0 and hi + lo > 0");
     ADRT_REQUIRE(d->max_batch >= 1, "plan_create: max_batch must be >= 1");
+    ADRT_REQUIRE(d->refine >= 0, "plan_create: refine must be >= 0");
     ADRT_REQUIRE(d->z && d->deg && d->a && d->c, "plan_create: null row parameter array");
     ADRT_REQUIRE(d->n_dense >= 0 && d->n_dense <= (2 << d->n), "plan_create: bad n_dense");
     ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && d->dense_F), "plan_create: dense rows need dense_k and dense_F");
@@ -199,6 +221,45 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
             (st = up(p->dFT, ft.data(), sizeof(c128) * ft.size())))
             return fail(st);
     }
+    // pass matrices of the dense rows (stored transposed for coalesced streaming)
+    if (d->n_dense > 0 && d->dense_Wop && d->dense_Wback) {
+        const size_t nn = (size_t)N * N;
+        std::vector<c128> t((size_t)d->n_dense * nn);
+        auto transpose_up = [&](const adrt_c128* W, c128** dst) -> int {
+            const c128* w = (const c128*)W;
+            for (int s = 0; s < d->n_dense; ++s)
+                for (int i = 0; i < N; ++i)
+                    for (int j = 0; j < N; ++j) t[s * nn + (size_t)j * N + i] = w[s * nn + (size_t)i * N + j];
+            int r = alloc((void**)dst, sizeof(c128) * t.size());
+            return r ? r : up(*dst, t.data(), sizeof(c128) * t.size());
+        };
+        if ((st = transpose_up(d->dense_Wop, &p->WopT)) || (st = transpose_up(d->dense_Wback, &p->WbackT)))
+            return fail(st);
+        if ((st = alloc((void**)&p->dense_k, sizeof(int) * d->n_dense)) ||
+            (st = up(p->dense_k, d->dense_k, sizeof(int) * d->n_dense)))
+            return fail(st);
+    }
+    p->d.dense_Wop = p->d.dense_Wback = nullptr;
+    // block order of the row kernel: longest (dense first, then highest degree) frequencies first
+    {
+        std::vector<int> ord(L);
+        std::vector<long long> key(L);
+        for (int k = 0; k < L; ++k) {
+            ord[k] = k;
+            key[k] = (long long)d->deg[k];
+        }
+        for (int i = 0; i < d->n_dense; ++i) key[d->dense_k[i]] = 1LL << 40;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[x] > key[y]; });
+        if ((st = alloc((void**)&p->korder, sizeof(int) * L)) || (st = up(p->korder, ord.data(), sizeof(int) * L)))
+            return fail(st);
+        // non-dense frequencies follow the dense ones in ord
+        const int nd = d->n_dense;
+        if (nd < L) {
+            if ((st = alloc((void**)&p->korder_nd, sizeof(int) * (L - nd))) ||
+                (st = up(p->korder_nd, ord.data() + nd, sizeof(int) * (L - nd))))
+                return fail(st);
+        }
+    }
     // FFT twiddles exp(-2 pi i m / L) for m < N, from the exact angle in double
     {
         std::vector<c128> t64(N);
@@ -208,14 +269,15 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
             t64[m] = mk(std::cos(ang), std::sin(ang));
             t32[m] = mk((float)t64[m].x, (float)t64[m].y);
         }
-        const size_t tb = twiddle_bytes(n, d->precision);
-        if ((st = alloc(&p->tw, tb)) || (st = up(p->tw, d->precision == PREC_F64 ? (const void*)t64.data()
-                                                                                 : (const void*)t32.data(), tb)))
+        if ((st = alloc(&p->tw, twiddle_bytes(n, PREC_F64))) || (st = up(p->tw, t64.data(), twiddle_bytes(n, PREC_F64))))
+            return fail(st);
+        if ((st = alloc(&p->tw32, twiddle_bytes(n, PREC_F32))) ||
+            (st = up(p->tw32, t32.data(), twiddle_bytes(n, PREC_F32))))
             return fail(st);
     }
     const size_t S = d->max_batch, NN = (size_t)N * N;
-    if ((st = alloc(&p->Y, S * 2 * L * N * row_elem(d->precision)))) return fail(st);
-    if ((st = alloc(&p->Sq, S * 4 * NN * strip_elem(d->precision)))) return fail(st);
+    if ((st = alloc(&p->Y, S * 2 * L * N * row_elem(PREC_F64)))) return fail(st);
+    if ((st = alloc(&p->Sq, S * 4 * NN * strip_elem(PREC_F64)))) return fail(st);
     if ((st = alloc((void**)&p->y, S * NN * sizeof(double)))) return fail(st);
     if ((st = alloc((void**)&p->xs, S * NN * sizeof(double)))) return fail(st);
     if ((st = alloc((void**)&p->res, S * NN * sizeof(double)))) return fail(st);
@@ -266,7 +328,7 @@ int adrt_plan_backproject(adrt_plan* plan, const double* d_b, double* d_y, int S
     ADRT_TRY(check_batch(p, S));
     if (S == 0) return OK;
     ADRT_REQUIRE(d_b && d_y, "plan_backproject: null buffer");
-    return packed_pass(p, 1, d_b, p->ops_back, p->nback, nullptr, d_y, S, (hipStream_t)stream);
+    return packed_pass(p, p->d.precision, 1, d_b, p->ops_back, p->nback, nullptr, d_y, S, (hipStream_t)stream);
 }
 
 int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream) {
@@ -274,7 +336,7 @@ int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, v
     ADRT_TRY(check_batch(p, S));
     if (S == 0) return OK;
     ADRT_REQUIRE(d_x && d_y, "plan_operator: null buffer");
-    return packed_pass(p, 0, d_x, p->ops_op, p->nop, nullptr, d_y, S, (hipStream_t)stream);
+    return packed_pass(p, p->d.precision, 0, d_x, p->ops_op, p->nop, nullptr, d_y, S, (hipStream_t)stream);
 }
 
 int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream) {
@@ -292,9 +354,9 @@ int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream) {
     ADRT_TRY(check_batch(p, S));
     ADRT_REQUIRE(which == 0 || which == 1, "plan_rows_pass: which must be 0 (backproject ops) or 1 (operator ops)");
     if (S == 0) return OK;
-    if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(p->d.precision)));
-    return which == 0 ? rows_pass(p, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2)
-                      : rows_pass(p, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2);
+    if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
+    return which == 0 ? rows_pass(p, p->d.precision, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2, p->WbackT)
+                      : rows_pass(p, p->d.precision, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2, p->WopT);
 }
 
 // x = outer Chebyshev of degree D on P G with right-hand side y = backproject(b) (fastops.py:247-258, 287-288, 337-338)
@@ -307,24 +369,39 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     const long long count = (long long)S * p->N * p->N;
     const double c = (p->d.hi - p->d.lo) / (p->d.hi + p->d.lo), a = 2.0 / (p->d.hi + p->d.lo);
     const int D = p->d.outer_deg;
-    ADRT_TRY(packed_pass(p, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st));
+    // FP32 plans with refine > 0: the right-hand side and the last `refine` outer steps in FP64 (a mixed-precision
+    // refinement: the FP64 steps pull the iterate to the FP64 operator's fixed point)
+    const int prec = p->d.precision;
+    const int refine = (prec == PREC_F32) ? (p->d.refine < D ? p->d.refine : D - 1) : 0;
+    ADRT_TRY(packed_pass(p, refine > 0 ? PREC_F64 : prec, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st));
     // iterate x_j lands in buffer j % 2, with buffer (D - 1) % 2 == d_x
     double* X[2];
     X[(D - 1) % 2] = d_x;
     X[D % 2] = p->xs;
     const bool has_p = !(p->d.kind == ADRT_KIND_LP || p->d.precond == ADRT_PRECOND_NONE);
+    // g1_inv fused with the outer scale / update when its degree fits the fused kernel's halo
+    const bool fused = has_p && p->d.precond == ADRT_PRECOND_G1 && p->d.g1_deg - 1 <= 16;
     const double* py = p->y;
-    if (has_p) {
-        ADRT_TRY(precond(p, p->y, p->r, S, st));
-        py = p->r;
+    if (fused) {
+        ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, p->y, nullptr, nullptr, a, 0.0, 0.0, X[0], S, st));
+    } else {
+        if (has_p) {
+            ADRT_TRY(precond(p, p->y, p->r, S, st));
+            py = p->r;
+        }
+        ADRT_TRY(launch_scale(py, a, X[0], count, st));
     }
-    ADRT_TRY(launch_scale(py, a, X[0], count, st));
     double om = 1.0;
     for (int k = 1; k < D; ++k) {
         om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
         const double* xk = X[(k - 1) % 2];
         const double* xp = (k == 1) ? nullptr : X[k % 2];
-        ADRT_TRY(packed_pass(p, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st));
+        const int pk = (refine > 0 && k >= D - refine) ? PREC_F64 : prec;
+        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st));
+        if (fused) {
+            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, p->res, xk, xp, a, om, om - 1.0, X[k % 2], S, st));
+            continue;
+        }
         const double* rr = p->res;
         if (has_p) {
             ADRT_TRY(precond(p, p->res, p->r, S, st));

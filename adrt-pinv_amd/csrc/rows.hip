@@ -1,4 +1,6 @@
 // Fused row kernel (see rows.hpp for the in-place index scheme) and its host launcher.
+#include <cstdlib>
+
 #include "rows.hpp"
 
 namespace adrt {
@@ -101,6 +103,7 @@ __global__ __launch_bounds__(NT) void rows_kernel(RowsArgs A) {
 
     for (long long row = blockIdx.x; row < A.rows; row += gridDim.x) {
         const int k = (int)(row % A.period);
+        if (A.skip_dense && A.dslot && A.dslot[k] >= 0) continue;  // block-uniform: one row per block
         const c128 zk = A.z[k];
         // twiddles z^t = prod over set bits j of t of z^(2^j), in double
         if (tid == 0) {
@@ -258,6 +261,29 @@ int dispatch(const RowsArgs& A, hipStream_t s) {
 
 }  // namespace
 
+// Per-(n, precision) choice of the register kernel's elements per lane: 0 = built-in default, -1 = generic kernel,
+// 2..4 = log2 elements per lane.  Set by the autotuner (adrt_rows_config_set) or the environment variable
+// ADRT_ROWS_LOGE (one value for every size, read once).
+static int g_rows_cfg[2][12];
+static int g_rows_env = -100;
+
+int rows_config_get(int n, int prec) {
+    if (g_rows_env == -100) {
+        const char* e = getenv("ADRT_ROWS_LOGE");
+        g_rows_env = e ? atoi(e) : 0;
+    }
+    if (n < 0 || n > 11 || prec < 0 || prec > 1) return 0;
+    if (g_rows_cfg[prec][n] != 0) return g_rows_cfg[prec][n];
+    return g_rows_env;
+}
+
+int rows_config_set(int n, int prec, int loge) {
+    ADRT_REQUIRE(n >= 0 && n <= 11 && (prec == 0 || prec == 1), "rows_config_set: bad n or precision");
+    ADRT_REQUIRE(loge >= -1 && loge <= 4 && loge != 1, "rows_config_set: loge must be -1, 0, 2, 3 or 4");
+    g_rows_cfg[prec][n] = loge;
+    return OK;
+}
+
 int check_ops(int n, const RowOp* ops, int nops) {
     ADRT_REQUIRE(nops >= 0 && nops <= MAX_OPS, "op list longer than 8");
     for (int i = 0; i < nops; ++i) {
@@ -272,6 +298,15 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s) {
     ADRT_REQUIRE(A.n >= 1 && A.n <= 11 && A.N == (1 << A.n), "rows: need 1 <= n <= 11 and N = 2^n");
     ADRT_TRY(check_ops(A.n, A.ops, A.nops));
     ADRT_REQUIRE(A.period >= 1, "rows: period must be positive");
+    if (A.loge >= 0 && rows_config_get(A.n, prec) >= 0) {
+        const int cfg = rows_config_get(A.n, prec);
+        const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));
+        if (loge > 0) {
+            const int st = launch_rows_reg(A, prec, loge, s);
+            if (st == OK) return OK;
+            if (st != E_INVALID) return st;
+        }
+    }
     if (prec == PREC_F64) return dispatch<double, double>(A, s);
     if (prec == PREC_F32) return dispatch<float, float>(A, s);
     set_error("rows: unknown precision");

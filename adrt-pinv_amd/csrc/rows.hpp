@@ -29,11 +29,29 @@ struct RowsArgs {
     const c128* dFT;        // dense tables, slot s at dFT + s*N*N, stored transposed: dFT[j*N + i] = F[i][j]
     int nops;
     RowOp ops[MAX_OPS];
+    // block mapping of the register kernel: group_by_k != 0 means rows = SP * period and a block owns rows of one
+    // frequency (equal degree); korder (may be null) lists the frequencies in block order (descending degree).
+    int group_by_k = 0;
+    int SP = 1;
+    const int* korder = nullptr;
+    int nk = 0;          // frequencies listed in korder (0 = period)
+    int skip_dense = 0;  // rows of dense frequencies are left untouched (a dense GEMM handles them)
+    int loge = 0;  // register kernel elements per lane (log2); 0 = default for (n, precision), -1 = generic kernel
 };
 
 // Host launcher (rows.hip).  prec selects the compute type of the butterflies and Chebyshev steps; the global row
 // type is complex128 for PREC_F64 and complex64 for PREC_F32.
 int launch_rows(const RowsArgs& args, int prec, hipStream_t stream);
+
+// Register-resident kernel (rows_reg.hip); E_INVALID if the op list or size needs the generic kernel.
+int launch_rows_reg(const RowsArgs& args, int prec, int loge, hipStream_t stream);
+int default_loge(int n, int prec);
+int rows_config_get(int n, int prec);
+int rows_config_set(int n, int prec, int loge);
+
+// Dense rows as a GEMM with the per-frequency pass matrices W^T (dense.hip).
+int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
+                      int prec, hipStream_t st);
 
 // Validates an op list against n; returns OK or E_INVALID with the error set.
 int check_ops(int n, const RowOp* ops, int nops);

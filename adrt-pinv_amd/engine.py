@@ -36,6 +36,29 @@ def level_steps(ops, rows, period, deg, dense_rows, n, N):
     return steps
 
 
+def level_matrices(n, z, ks, lo, hi):
+    """A_k[s, t] = (B_{lo..hi} e_t)[s] for the frequencies ks, from the GPU butterflies applied to the identity."""
+    N = 1 << n
+    ks = np.asarray(ks, dtype=np.int64)
+    X = np.tile(np.eye(N, dtype=complex), (ks.size, 1))
+    zr = np.ascontiguousarray(np.repeat(np.asarray(z)[ks], N))
+    L.check(L.load().adrt_bfly_rows(n, X.shape[0], L.ptr(zr), L.ptr(X), lo, hi, 0), "bfly_rows")
+    return X.reshape(ks.size, N, N).transpose(0, 2, 1)
+
+
+def pass_matrices(n, z, ks, F, kind):
+    """The whole op list of a dense row as one matrix per frequency (the ops are linear and row-local):
+    hybrid G_h [B1, P, B', B'^H, B1^H] -> B1^H A'^H A' F B1,  backprojection [B'^H, P, B1^H] -> B1^H F A'^H;
+    L_p frame [P, B, B^H] -> A^H A F,  backprojection [B^H, P] -> F A^H  (op lists of fastops.py:281-335)."""
+    H = lambda M: np.conj(M.transpose(0, 2, 1))
+    if kind == "hybrid":
+        B1 = level_matrices(n, z, ks, 1, 1)
+        Ap = level_matrices(n, z, ks, 2, n)
+        return H(B1) @ (H(Ap) @ Ap) @ F @ B1, H(B1) @ F @ H(Ap)
+    A = level_matrices(n, z, ks, 1, n)
+    return H(A) @ A @ F, F @ H(A)
+
+
 def dense_tables(n, z, tau, ks, lo, hi):
     """Exact Tikhonov filters F_k = (M_k + tau)^{-1}, M_k = A_k^H A_k over levels lo..hi, by eigh with the
     eigenvalues clipped at 0 (fastops.py:28-39).  A_k comes from the GPU butterflies applied to the identity."""

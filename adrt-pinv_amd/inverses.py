@@ -11,6 +11,7 @@ eigensolve) touches the host.  Extra keywords:
              preconditioner and outer iteration stay in double)
   max_batch  sinograms per device call (``apply_batch``)
   outer      (lo, hi, D) to skip the eigensolve (e.g. to reuse the reference's setup scalars)
+  refine     with precision 'fp32': the backprojection and the last ``refine`` outer steps run in FP64
 """
 
 import ctypes
@@ -40,16 +41,27 @@ class _PlanBase:
         self.eng = E.HipEngine(self.n, self.z, self.deg, self.lam_max, self.tau, self.threads, copies=2,
                                dense_cap=dense_cap, p_levels=p_levels, precision=self.precision)
 
+    # dense rows run as a batched GEMM with precomputed pass matrices when these fit this many bytes
+    DENSE_GEMM_BYTES = 8 << 30
+
     def _make_plan(self, lo=1.0, hi=2.0, D=1):
         e = self.eng
         self._keep = [np.ascontiguousarray(e.z), np.ascontiguousarray(e.deg, dtype=np.int32), e.a, e.c]
+        Wop = Wback = None
         if e.n_dense:
             self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), np.ascontiguousarray(e.dense_F)]
+            if 2 * e.n_dense * self.N * self.N * 16 <= self.DENSE_GEMM_BYTES:
+                kind = "hybrid" if self.KIND == L.KIND_HYBRID else "lp"
+                Wop, Wback = E.pass_matrices(self.n, self.z, e.dense_k, e.dense_F, kind)
+                Wop, Wback = np.ascontiguousarray(Wop), np.ascontiguousarray(Wback)
+                self._keep += [Wop, Wback]
         d = L.PlanDesc()
         d.kind = self.KIND
         d.n = self.n
         d.precision = L.PRECISION[self.precision]
-        d.precond = L.PRECOND[getattr(self, "precond", "none")]
+        # fastops.FastHybrid.P applies lorr_inv for every precond other than 'g1' (fastops.py:322-323)
+        pc = getattr(self, "precond", None)
+        d.precond = L.PRECOND["none"] if pc is None else L.PRECOND["g1" if pc == "g1" else "lorr"]
         d.g1_deg = int(getattr(self, "g1_deg", 1))
         d.outer_deg = int(D)
         d.lo, d.hi, d.tau = float(lo), float(hi), float(self.tau)
@@ -59,6 +71,9 @@ class _PlanBase:
         d.dense_k = L.ptr(self._keep[4]) if e.n_dense else None
         d.dense_F = L.ptr(self._keep[5]) if e.n_dense else None
         d.max_batch = int(self.max_batch)
+        d.dense_Wop = L.ptr(Wop) if Wop is not None else None
+        d.dense_Wback = L.ptr(Wback) if Wback is not None else None
+        d.refine = int(self.refine)
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
@@ -169,9 +184,10 @@ class FastLpHip(_PlanBase):
     KIND = L.KIND_LP
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None,
-                 power_iters=40, safety=1.5, dense_cap=None, precision="fp64", max_batch=1, outer=None):
+                 power_iters=40, safety=1.5, dense_cap=None, precision="fp64", max_batch=1, outer=None, refine=0):
         self.precision = precision
         self.max_batch = max_batch
+        self.refine = refine
         self._setup_common(n, tau_fac, eps_in, threads)
         if lam_max is None:
             lam_max = {
@@ -204,9 +220,11 @@ class FastHybridHip(_PlanBase):
     KIND = L.KIND_HYBRID
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None, g1_deg=14,
-                 power_iters=40, safety=1.5, dense_cap=None, precond="g1", precision="fp64", max_batch=1, outer=None):
+                 power_iters=40, safety=1.5, dense_cap=None, precond="g1", precision="fp64", max_batch=1, outer=None,
+                 refine=0):
         self.precision = precision
         self.max_batch = max_batch
+        self.refine = refine
         self.g1_deg = g1_deg
         self.precond = precond
         self._setup_common(n, tau_fac, eps_in, threads)

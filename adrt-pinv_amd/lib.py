@@ -44,6 +44,8 @@ SIGNATURES = {
     "adrt_event_destroy": (I, [P]),
     "adrt_event_record": (I, [P, P]),
     "adrt_event_elapsed_ms": (I, [P, P, ctypes.POINTER(ctypes.c_float)]),
+    "adrt_rows_config_set": (I, [I, I, I]),
+    "adrt_rows_config_get": (I, [I, I]),
     "adrt_rows_plan_create": (I, [ctypes.POINTER(P), I, I, P, P, P, P, D, I, P, P, I]),
     "adrt_rows_plan_destroy": (I, [P]),
     "adrt_rows_run": (I, [P, P, P, LL, P, I, P]),
@@ -75,6 +77,7 @@ class PlanDesc(ctypes.Structure):
         ("lo", D), ("hi", D), ("tau", D),
         ("lam_max", P), ("z", P), ("deg", P), ("a", P), ("c", P),
         ("n_dense", I), ("dense_k", P), ("dense_F", P), ("max_batch", I),
+        ("dense_Wop", P), ("dense_Wback", P), ("refine", I),
     ]
 
 

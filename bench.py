@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--n", type=int, default=8)
     ap.add_argument("--dense-cap", type=int, default=16)
+    ap.add_argument("--refine", type=int, default=2, help="FP64 refinement steps at the end of the FP32 apply")
     ap.add_argument("--plain-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -107,7 +108,7 @@ def main():
     H64 = I.FastHybridHip(n, dense_cap=cap, precision="fp64", max_batch=1)
     setup_s = time.time() - t0
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B,
-                          outer=(H64.lo, H64.hi, H64.outer_deg))
+                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=a.refine)
     print(f"# setup {setup_s:.1f}s: D={H64.outer_deg} sum(deg)/N={H64.deg.sum() / N:.1f} dense rows={H64.eng.n_dense}",
           file=sys.stderr, flush=True)
 
@@ -192,7 +193,7 @@ def main():
         "dtype": "fp32 rows + fp64 dense/precond/outer",
         "data": "synthetic: ellipse phantoms (seeds 1000+i) + 1% Gaussian noise (seeds 2000+i)",
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
-                   "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg,
+                   "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
                    "sum_deg_over_N": float(H64.deg.sum() / N), "n_dense": H64.eng.n_dense},
         "plain_fp64_inverses_per_s": plain_ips,
         "speedup_vs_plain_fp64": ips / plain_ips,

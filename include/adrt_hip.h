@@ -82,6 +82,11 @@ int adrt_rows_plan_destroy(adrt_rows_plan* plan);
 int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows, const int* ops, int nops,
                   void* stream);
 
+/* Row-kernel variant per (n, precision): 0 = built-in default, -1 = generic LDS kernel, 2..4 = register kernel with
+ * 2^loge values per lane.  Chosen by the autotuner; the environment variable ADRT_ROWS_LOGE sets every size. */
+int adrt_rows_config_set(int n, int precision, int loge);
+int adrt_rows_config_get(int n, int precision);
+
 /* Inverse plans: fastops.FastLp / FastHybrid (fastops.py:261-338) with the setup scalars given.
  * All image / sinogram buffers are float64 on the device:
  *   b: (S, 4, 2N-1, N) sinograms, x / y: (S, N, N) images. */
@@ -107,6 +112,12 @@ typedef struct {
     const int* dense_k; /* (n_dense) their indices in [0, L) */
     const adrt_c128* dense_F; /* (n_dense, N, N) row-major */
     int max_batch;      /* sinograms per call; scratch is sized for it */
+    /* optional: the whole op list of a dense row as one matrix (row-major (n_dense, N, N)): the operator pass
+       (B1^H M' F B1 for G_h, M F for the frame) and the backprojection pass (B1^H F A'^H, F A^H).  When given,
+       dense rows run as a batched GEMM instead of inside the row kernel. */
+    const adrt_c128* dense_Wop;
+    const adrt_c128* dense_Wback;
+    int refine;         /* ADRT_PREC_F32 only: the backprojection and the last `refine` outer steps run in FP64 */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;

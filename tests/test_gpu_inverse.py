@@ -66,3 +66,93 @@ def test_left_inverse(gpu):
     d = golden("inverse_n6_phantom")
     H = I.FastHybridHip(n, lam_max=d["h_lam_max"], outer=(float(d["h_lo"]), float(d["h_hi"]), int(d["h_D"])))
     assert np.linalg.norm(H.apply(b) - x) / np.linalg.norm(x) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["hybrid", "lp"])
+@pytest.mark.parametrize("n,cap", [(6, 8), (7, 10)])
+def test_dense_rows_vs_oracle(gpu, kind, n, cap):
+    """Dense low-frequency rows (exact Tikhonov filter, run as a batched GEMM of the pass matrices) against the
+    oracle's sequential ops with its own eigh tables."""
+    from oracle import inverses as oiv
+    from oracle import radon as orad
+
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    N = 1 << n
+    x = ph.ellipses(11, N)
+    b = ph.add_noise(orad.adrt(n, x), 0.01, 12)
+    z = np.exp(1j * 2 * np.pi * np.fft.fftfreq(2 * N))
+    if kind == "hybrid":
+        lam = oiv.power_bound(n, z, 2, streams="numpy")
+        R = oiv.Hybrid(n, lam_max=lam, dense_cap=cap, outer=(0.16, 0.96, 30))
+        G = I.FastHybridHip(n, lam_max=lam, dense_cap=cap, outer=(0.16, 0.96, 30), max_batch=2)
+        assert relmax(G.Gh(x), R.Gh(x)) < 1e-12
+    else:
+        lam = oiv.power_bound(n, z, 1, streams="numpy")
+        R = oiv.Lp(n, lam_max=lam, dense_cap=cap, outer=(1.0, 4.0, 25))
+        G = I.FastLpHip(n, lam_max=lam, dense_cap=cap, outer=(1.0, 4.0, 25), max_batch=2)
+        assert relmax(G.frame_window(x), R.frame_window(x)) < 1e-12
+    assert G.eng.n_dense == R.eng.n_dense > 0
+    assert relmax(G.backproject(b), R.backproject(b)) < 1e-12
+    assert relmax(G.apply(b), R.apply(b)) < 1e-12
+    xb = G.apply_batch(np.stack([b, 2 * b]))
+    assert relmax(xb[1], 2 * xb[0]) < 1e-12  # linear in b
+
+
+def test_lorr_and_none_preconditioners(gpu):
+    from oracle import inverses as oiv
+    from oracle import radon as orad
+
+    d = golden("inverse_n6_phantom")
+    n = 6
+    I = pkg("inverses")
+    b = orad.adrt(n, d["x"])
+    for pc, outer in (("lorr", (0.15, 0.95, 30)), ("none", (0.5, 15.0, 40))):
+        R = oiv.Hybrid(n, lam_max=d["h_lam_max"], precond=pc, outer=outer)
+        G = I.FastHybridHip(n, lam_max=d["h_lam_max"], precond=pc, outer=outer)
+        assert relmax(G.P(d["x"]), R.P(d["x"])) < 1e-13  # fastops applies lorr_inv for any precond but 'g1'
+        assert relmax(G.apply(b), R.apply(b)) < 1e-12, pc
+
+
+def test_fp32_and_refinement(gpu):
+    """FP32 row engine against FP64; the FP64 refinement steps pull the FP32 result onto the FP64 one."""
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    n = 7
+    N = 1 << n
+    H = I.FastHybridHip(n, dense_cap=12, max_batch=2)
+    outer = (H.lo, H.hi, H.outer_deg)
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(20 + i, N)), 0.01, 30 + i) for i in range(2)])
+    x64 = H.apply_batch(B)
+    err = {}
+    for refine in (0, 3):
+        H32 = I.FastHybridHip(n, lam_max=H.lam_max, dense_cap=12, precision="fp32", max_batch=2, outer=outer,
+                              refine=refine)
+        x32 = H32.apply_batch(B)
+        err[refine] = max(np.linalg.norm(x32[i] - x64[i]) / np.linalg.norm(x64[i]) for i in range(2))
+    assert err[0] < 1e-4
+    assert err[3] < 1e-6 and err[3] < err[0] / 10, err
+
+
+def test_graph_replay_matches_eager(gpu):
+    """A captured apply replays to the eager result, with eager work and syncs between replays."""
+    L = gpu
+    I = pkg("inverses")
+    d = golden("inverse_n6_phantom")
+    H = I.FastHybridHip(6, lam_max=d["h_lam_max"], outer=(float(d["h_lo"]), float(d["h_hi"]), int(d["h_D"])),
+                        max_batch=3, dense_cap=10)
+    B = np.stack([d["b"], 0.5 * d["b"], -d["b"]])
+    eager = H.apply_batch(B)
+    st = L.Stream()
+    db = L.DeviceArray.from_host(B, st)
+    dx = L.DeviceArray((3, 64, 64))
+    with L.Graph(st) as g:
+        H.apply_device(db, dx, 3, st, count=False)
+    for rep in range(3):
+        dx.zero(st)
+        g.launch(st)
+        st.sync()
+        got = dx.download(st)
+        assert relmax(got, eager) < 1e-13, rep
+        H.apply(d["b"])  # eager work between replays
