@@ -1,0 +1,200 @@
+// Offset-axis FFTs of length L = 32 * L1 (L = 128 .. 1024) as a two-step (four-step) FFT held in registers.
+//
+//   X[k2 + 32 k1] = sum_h1 W_L^{h1 k2} W_L1^{h1 k1} sum_h2 x[h1 + L1 h2] W_32^{h2 k2}
+//
+// A block owns C columns of one (sinogram, pair) with L1 threads per column.  Step 1: thread (h1, c) loads the 32
+// values x[h1 + L1 h2] of its column (fused with the quadrant packing / iota of the strips), does a 32-point DFT in
+// registers and applies W_L^{h1 k2}.  One LDS exchange regroups the values by k2.  Step 2: each thread does 32 / L1
+// DFTs of length L1 over h1 and writes X[k2 + 32 k1] (the inverse variant splits Re / Im into the quadrant strips and
+// keeps rows h < N).  Global reads and writes move C consecutive columns per row.  Twiddles come from the plan's
+// table W_L^m (m < L/2) staged in LDS.
+#include "passes.hpp"
+
+namespace adrt {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ cplx<T> tw_at(const cplx<T>* tw, int m, int L) {
+    // W_L^m for 0 <= m < L from the half table (W^(m + L/2) = -W^m)
+    const int h = L >> 1;
+    cplx<T> w = tw[m & (h - 1)];
+    if (m & h) w = mk(-w.x, -w.y);
+    return w;
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> rot(cplx<T> w, cplx<T> v) {
+    return INV ? cmulc(w, v) : mul(w, v);  // forward W = exp(-i..), inverse conj(W)
+}
+
+// In-place radix-2 decimation-in-frequency DFT of M points in registers: result index k sits at position
+// bitrev_log2M(k).  w32[m] = W_32^m (m < 16) in the transform's sign convention (conjugated by rot for INV).
+template <typename T, int M, bool INV>
+__device__ __forceinline__ void dft_dif(cplx<T> (&v)[M], const cplx<T>* w32) {
+#pragma unroll
+    for (int half = M / 2; half >= 1; half >>= 1) {
+#pragma unroll
+        for (int a0 = 0; a0 < M; a0 += 2 * half) {
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const cplx<T> x0 = v[a0 + j], x1 = v[a0 + j + half];
+                v[a0 + j] = add(x0, x1);
+                const cplx<T> d = sub(x0, x1);
+                // W_{2 half}^j = W_32^(j * 16 / half)
+                v[a0 + j + half] = (j == 0) ? d : rot<T, INV>(w32[j * (16 / half)], d);
+            }
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ constexpr int brev_c(int x) {
+    int r = 0;
+    for (int b = 1; b < M; b <<= 1) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ double image_strip2(const double* x, int N, int q, int h, int col) {
+    switch (q) {
+        case 0: return x[(size_t)col * N + (N - 1 - h)];
+        case 1: return x[(size_t)(N - 1 - h) * N + col];
+        case 2: return x[(size_t)h * N + col];
+        default: return x[(size_t)(N - 1 - col) * N + (N - 1 - h)];
+    }
+}
+
+// MODE 0: pack image strips (forward), 1: pack data strips (forward), 2: inverse + quadrant split
+template <typename Tg, typename T, int L1, int C, int MODE>
+__global__ __launch_bounds__(L1* C) void fft2_kernel(int n, const void* src, void* dst, const cplx<T>* twg) {
+    constexpr int L = 32 * L1;
+    constexpr int P = C + 1;  // LDS pitch per row of the exchange
+    constexpr bool INV = MODE == 2;
+    constexpr int KPT = 32 / L1;  // k2 values per thread in step 2
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* S = reinterpret_cast<cplx<T>*>(smem_raw);  // L * P values
+    cplx<T>* tw = S + L * P;                              // L/2 twiddles
+    const int N = 1 << n;
+    const int t = threadIdx.x;
+    const int cl = t % C, h1 = t / C;
+    const int c = blockIdx.x * C + cl;
+    const int pair = blockIdx.y, s = blockIdx.z;
+    for (int i = t; i < L / 2; i += L1 * C) tw[i] = twg[i];
+    __syncthreads();
+    cplx<T> w32[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) w32[m] = tw[m * L1];
+
+    // ---- step 1: load x[h1 + L1 h2], 32-point DFT over h2, twiddle W_L^{h1 k2}
+    cplx<T> v[32];
+    if (MODE == 0) {
+        const double* x = reinterpret_cast<const double*>(src) + (size_t)s * N * N;
+#pragma unroll
+        for (int h2 = 0; h2 < 32; ++h2) {
+            const int h = h1 + L1 * h2;
+            if (h2 < 16) {  // h < N (N = 16 L1); the zero padding rows h >= N contribute nothing
+                v[h2].x = (T)image_strip2(x, N, 2 * pair, h, c);
+                v[h2].y = (T)image_strip2(x, N, 2 * pair + 1, h, c);
+            } else {
+                v[h2] = mk((T)0, (T)0);
+            }
+        }
+    } else if (MODE == 1) {
+        const size_t qsz = (size_t)(2 * N - 1) * N;
+        const double* ba = reinterpret_cast<const double*>(src) + ((size_t)s * 4 + 2 * pair) * qsz;
+        const double* bb = ba + qsz;
+#pragma unroll
+        for (int h2 = 0; h2 < 32; ++h2) {
+            const int h = h1 + L1 * h2;
+            if (h == N) {
+                v[h2] = mk((T)0, (T)0);
+            } else {
+                const int r = h < N ? h + N - 1 : h - N - 1;  // sinogram row of torus row h
+                v[h2] = mk((T)ba[(size_t)r * N + c], (T)bb[(size_t)r * N + c]);
+            }
+        }
+    } else {
+        const cplx<Tg>* Z = reinterpret_cast<const cplx<Tg>*>(src) + ((size_t)s * 2 + pair) * L * N;
+#pragma unroll
+        for (int h2 = 0; h2 < 32; ++h2) v[h2] = cast<T>(Z[(size_t)(h1 + L1 * h2) * N + c]);
+    }
+    dft_dif<T, 32, INV>(v, w32);
+    // v[brev5(k2)] holds the sum for k2; apply W_L^{h1 k2} and store to S[(h1 + L1 k2) * P + cl]
+#pragma unroll
+    for (int k2 = 0; k2 < 32; ++k2) {
+        cplx<T> y = v[brev_c<32>(k2)];
+        if (k2 != 0 && h1 != 0) y = rot<T, INV>(tw_at(tw, h1 * k2, L), y);
+        S[(h1 + L1 * k2) * P + cl] = y;
+    }
+    __syncthreads();
+
+    // ---- step 2: thread (g, cl) owns k2 = g * KPT + u; L1-point DFT over h1
+    const int g = h1;
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+        const int k2 = g * KPT + u;
+        cplx<T> w[L1];
+#pragma unroll
+        for (int a = 0; a < L1; ++a) w[a] = S[(a + L1 * k2) * P + cl];
+        dft_dif<T, L1, INV>(w, w32);
+#pragma unroll
+        for (int k1 = 0; k1 < L1; ++k1) {
+            const cplx<T> X = w[brev_c<L1>(k1)];
+            const int k = k2 + 32 * k1;
+            if (MODE != 2) {
+                cplx<Tg>* Y = reinterpret_cast<cplx<Tg>*>(dst) + ((size_t)s * 2 + pair) * L * N;
+                Y[(size_t)k * N + c] = cast<Tg>(X);
+            } else if (k < N) {
+                const T inv = (T)1 / (T)L;
+                Tg* Sa = reinterpret_cast<Tg*>(dst) + ((size_t)s * 4 + 2 * pair) * N * N;
+                Tg* Sb = Sa + (size_t)N * N;
+                Sa[(size_t)k * N + c] = (Tg)(X.x * inv);
+                Sb[(size_t)k * N + c] = (Tg)(X.y * inv);
+            }
+        }
+    }
+}
+
+template <typename Tg, typename T, int L1, int C, int MODE>
+int launch_cfg(int n, const void* src, void* dst, int S, const void* tw, hipStream_t st) {
+    constexpr int L = 32 * L1;
+    const size_t lds = sizeof(cplx<T>) * ((size_t)L * (C + 1) + L / 2);
+    const int N = 1 << n;
+    hipLaunchKernelGGL((fft2_kernel<Tg, T, L1, C, MODE>), dim3(N / C, 2, S), dim3(L1 * C), lds, st, n, src, dst,
+                       (const cplx<T>*)tw);
+    return launch_check("fft2_kernel");
+}
+
+template <typename Tg, typename T, int MODE>
+int launch_mode(int n, const void* src, void* dst, int S, const void* tw, hipStream_t st) {
+    constexpr bool F64 = sizeof(T) == 8;
+    switch (n) {
+        case 6: return launch_cfg<Tg, T, 4, (F64 ? 32 : 64), MODE>(n, src, dst, S, tw, st);
+        case 7: return launch_cfg<Tg, T, 8, (F64 ? 16 : 32), MODE>(n, src, dst, S, tw, st);
+        case 8: return launch_cfg<Tg, T, 16, (F64 ? 8 : 16), MODE>(n, src, dst, S, tw, st);
+        case 9: return launch_cfg<Tg, T, 32, (F64 ? 4 : 8), MODE>(n, src, dst, S, tw, st);
+        default: set_error("fft2: n must be 6..9"); return E_INVALID;
+    }
+}
+
+}  // namespace
+
+bool fft2_supported(int n) { return n >= 6 && n <= 9; }
+
+int launch_fft2(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st) {
+    if (S <= 0) return OK;
+    ADRT_REQUIRE(fft2_supported(n), "fft2: n must be 6..9");
+    if (prec == PREC_F64) {
+        if (mode == 0) return launch_mode<double, double, 0>(n, src, dst, S, tw, st);
+        if (mode == 1) return launch_mode<double, double, 1>(n, src, dst, S, tw, st);
+        return launch_mode<double, double, 2>(n, src, dst, S, tw, st);
+    }
+    if (mode == 0) return launch_mode<float, float, 0>(n, src, dst, S, tw, st);
+    if (mode == 1) return launch_mode<float, float, 1>(n, src, dst, S, tw, st);
+    return launch_mode<float, float, 2>(n, src, dst, S, tw, st);
+}
+
+}  // namespace adrt

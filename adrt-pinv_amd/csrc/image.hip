@@ -93,7 +93,8 @@ template <int G1_TILE>
 __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     constexpr int G1_REG = G1_TILE + 2 * G1_MAXH;
     constexpr int G1_RP = G1_REG + 2;  // LDS row pitch: region plus a zero border
-    constexpr int G1_PER = (G1_REG * G1_REG + G1_THREADS - 1) / G1_THREADS;
+    constexpr int G1_PER = G1_REG * G1_REG / G1_THREADS;
+    static_assert(G1_REG * G1_REG % G1_THREADS == 0, "region slots must tile the block");
     extern __shared__ double X[];  // G1_RP * G1_RP
     const int N = 1 << A.n;
     const int H = A.deg - 1;
@@ -111,11 +112,11 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
         const int idx = threadIdx.x + G1_THREADS * e;
         const int ri = idx / G1_REG, rj = idx % G1_REG;
         const int gi = ti0 + ri, gj = tj0 + rj;
-        const bool in = idx < G1_REG * G1_REG && ri < R && rj < R && gi >= 0 && gi < N && gj >= 0 && gj < N;
+        const bool in = ri < R && rj < R && gi >= 0 && gi < N && gj >= 0 && gj < N;
         inmask |= (in ? 1u : 0u) << e;
         yv[e] = in ? ys[(size_t)gi * N + gj] : 0.0;
         xp[e] = 0.0;
-        if (idx < G1_REG * G1_REG) X[(ri + 1) * G1_RP + rj + 1] = a * yv[e];
+        X[(ri + 1) * G1_RP + rj + 1] = a * yv[e];
     }
     __syncthreads();
     for (int k = 1; k < A.deg; ++k) {
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             // parity of the global index equals the parity of the region index shifted by the (even) origin offset
             const int pi = (ti0 + ri) & 1, pj = (tj0 + rj) & 1;
             // pixels outside the image hold 0 in X and the border row/column is 0, so every neighbour read is plain
-            const int c = idx < G1_REG * G1_REG ? (ri + 1) * G1_RP + rj + 1 : G1_RP + 1;
+            const int c = (ri + 1) * G1_RP + rj + 1;
             const int cj = c + (pj ? -1 : 1), ci = c + (pi ? -G1_RP : G1_RP);
             const double xk = X[c];
             const double t1 = 2.0 * X[cj] + X[cj - G1_RP] + X[cj + G1_RP];
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
         for (int e = 0; e < G1_PER; ++e) {
             const int idx = threadIdx.x + G1_THREADS * e;
             const int ri = idx / G1_REG, rj = idx % G1_REG;
-            if (idx < G1_REG * G1_REG) X[(ri + 1) * G1_RP + rj + 1] = xn[e];
+            X[(ri + 1) * G1_RP + rj + 1] = xn[e];
         }
         __syncthreads();
     }
@@ -153,8 +154,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
         const int idx = threadIdx.x + G1_THREADS * e;
         const int ri = idx / G1_REG, rj = idx % G1_REG;
         const int gi = ti0 + ri, gj = tj0 + rj;
-        if (idx >= G1_REG * G1_REG || ri < H || rj < H || ri >= H + G1_TILE || rj >= H + G1_TILE || gi >= N || gj >= N)
-            continue;
+        if (ri < H || rj < H || ri >= H + G1_TILE || rj >= H + G1_TILE || gi >= N || gj >= N) continue;
         const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
         const double pv = X[(ri + 1) * G1_RP + rj + 1];
         if (A.mode == 0) {

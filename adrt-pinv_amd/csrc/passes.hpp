@@ -19,6 +19,10 @@ int launch_ifft_unpack(int n, const void* Z, void* Sq, int S, int prec, const vo
 // If y != nullptr the kernel writes y - out instead (the outer residual).
 int launch_combine(int n, const void* Sq, int prec, const double* y, double* out, int S, hipStream_t st);
 
+// Two-step register FFT for 6 <= n <= 9 (fft2.hip); mode 0 / 1 as launch_pack_fft, mode 2 as launch_ifft_unpack.
+bool fft2_supported(int n);
+int launch_fft2(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st);
+
 // Twiddle table exp(sign * 2 pi i m / L), m < L/2, in the precision's type (host builds it once per plan).
 size_t twiddle_bytes(int n, int prec);
 

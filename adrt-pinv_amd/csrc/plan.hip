@@ -84,6 +84,10 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         A.nk = p->L - p->d.n_dense;
         A.skip_dense = 1;
         if (A.nk > 0) ADRT_TRY(launch_rows(A, prec, st));
+        if (p->d.n >= 4 && 2 * S > 4) {  // the MFMA tile (64 rows) pays off once several sinograms share W
+            if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
+            return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, p->Y2, prec, st);
+        }
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
     }
     return launch_rows(A, prec, st);
