@@ -213,23 +213,39 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
             }
             slot[d->dense_k[i]] = i;
         }
-        std::vector<c128> ft((size_t)d->n_dense * N * N);
-        const c128* F = (const c128*)d->dense_F;
-        for (int s = 0; s < d->n_dense; ++s)
-            for (int i = 0; i < N; ++i)
-                for (int j = 0; j < N; ++j)
-                    ft[(size_t)s * N * N + (size_t)j * N + i] = F[(size_t)s * N * N + (size_t)i * N + j];
         if ((st = alloc((void**)&p->dslot, sizeof(int) * L)) || (st = up(p->dslot, slot.data(), sizeof(int) * L)))
             return fail(st);
-        if ((st = alloc((void**)&p->dFT, sizeof(c128) * ft.size())) ||
-            (st = up(p->dFT, ft.data(), sizeof(c128) * ft.size())))
-            return fail(st);
+        const size_t tb = sizeof(c128) * (size_t)d->n_dense * N * N;
+        if ((st = alloc((void**)&p->dFT, tb))) return fail(st);
+        if (d->dense_on_device) {
+            if (hipMemcpy(p->dFT, d->dense_F, tb, hipMemcpyDeviceToDevice) != hipSuccess) {
+                set_error("plan_create: device copy of the dense tables failed");
+                return fail(E_HIP);
+            }
+        } else {
+            std::vector<c128> ft((size_t)d->n_dense * N * N);
+            const c128* F = (const c128*)d->dense_F;
+            for (int s = 0; s < d->n_dense; ++s)
+                for (int i = 0; i < N; ++i)
+                    for (int j = 0; j < N; ++j)
+                        ft[(size_t)s * N * N + (size_t)j * N + i] = F[(size_t)s * N * N + (size_t)i * N + j];
+            if ((st = up(p->dFT, ft.data(), tb))) return fail(st);
+        }
     }
     // pass matrices of the dense rows (stored transposed for coalesced streaming)
     if (d->n_dense > 0 && d->dense_Wop && d->dense_Wback) {
         const size_t nn = (size_t)N * N;
-        std::vector<c128> t((size_t)d->n_dense * nn);
+        std::vector<c128> t(d->dense_on_device ? 0 : (size_t)d->n_dense * nn);
         auto transpose_up = [&](const adrt_c128* W, c128** dst) -> int {
+            if (d->dense_on_device) {
+                int r = alloc((void**)dst, sizeof(c128) * d->n_dense * nn);
+                if (r) return r;
+                if (hipMemcpy(*dst, W, sizeof(c128) * d->n_dense * nn, hipMemcpyDeviceToDevice) != hipSuccess) {
+                    set_error("plan_create: device copy of the pass matrices failed");
+                    return E_HIP;
+                }
+                return OK;
+            }
             const c128* w = (const c128*)W;
             for (int s = 0; s < d->n_dense; ++s)
                 for (int i = 0; i < N; ++i)

@@ -106,18 +106,31 @@ class HipEngine:
         self.p_levels = (p_levels[0], p_levels[1] or n)
         self.dense_k = np.nonzero(self.deg > dense_cap)[0].astype(np.int32) if dense_cap is not None else np.zeros(0, np.int32)
         self.n_dense = int(self.dense_k.size)
-        self.dense_F = dense_tables(n, self.z, self.tau, self.dense_k, *self.p_levels) if self.n_dense else None
-        h = L.P()
-        L.check(
-            L.load().adrt_rows_plan_create(
-                ctypes.byref(h), n, self.L, L.ptr(self.z), L.ptr(self.deg), L.ptr(self.a), L.ptr(self.c), self.tau,
-                self.n_dense, L.ptr(self.dense_k) if self.n_dense else None,
-                L.ptr(np.ascontiguousarray(self.dense_F)) if self.n_dense else None, L.PRECISION[precision],
-            ),
-            "rows_plan_create",
-        )
-        self._plan = h.value
+        self._dense_F = None
+        self._plan = None
         self.count = [0]
+
+    @property
+    def dense_F(self):
+        """Host row-major exact filters of the dense frequencies (built on first use)."""
+        if self._dense_F is None and self.n_dense:
+            self._dense_F = dense_tables(self.n, self.z, self.tau, self.dense_k, *self.p_levels)
+        return self._dense_F
+
+    def _rows_plan(self):
+        if self._plan is None:
+            h = L.P()
+            F = np.ascontiguousarray(self.dense_F) if self.n_dense else None
+            L.check(
+                L.load().adrt_rows_plan_create(
+                    ctypes.byref(h), self.n, self.L, L.ptr(self.z), L.ptr(self.deg), L.ptr(self.a), L.ptr(self.c),
+                    self.tau, self.n_dense, L.ptr(self.dense_k) if self.n_dense else None, L.ptr(F),
+                    L.PRECISION[self.precision],
+                ),
+                "rows_plan_create",
+            )
+            self._plan = h.value
+        return self._plan
 
     def steps(self, ops, rows=None):
         return level_steps(ops, self.R if rows is None else rows, self.L, self.deg, self.dense_k, self.n, self.N)
@@ -125,7 +138,7 @@ class HipEngine:
     def run_device(self, dY, dZ, rows, ops, stream=None):
         """Rows on the device: dY -> dZ (may alias), complex128 (fp64) or complex64 (fp32)."""
         o = L.ops_array(ops)
-        L.check(L.load().adrt_rows_run(self._plan, L.ptr(dY), L.ptr(dZ), rows, L.ptr(o), len(ops),
+        L.check(L.load().adrt_rows_run(self._rows_plan(), L.ptr(dY), L.ptr(dZ), rows, L.ptr(o), len(ops),
                                        stream.handle if stream else None), "rows_run")
         self.count[0] += self.steps(ops, rows)
 

@@ -1,0 +1,140 @@
+"""Dense low-frequency tables built on the GPU (setup only): rocSOLVER batched Hermitian eigensolves (QR iteration)
+and rocBLAS batched complex GEMMs, reached through ctypes.
+
+For every dense frequency k (degree above dense_cap) the reference builds F_k = (M_k + tau)^{-1} with
+M_k = A_k^H A_k over the P levels, by eigh with the eigenvalues clipped at 0 (fastops.py:28-39).  The device plan also
+wants the whole op list of such a row as one matrix (dense.hip): for the hybrid G_h = B1^H M' F B1 and the
+backprojection B1^H F A'^H; for L_p M F and F A^H.  All matrices live on the device in column-major order, which is
+the row-major transpose the plan stores.  The level matrices come from the package's own butterflies applied to the
+identity: the rows it returns are the columns of A, i.e. A in column-major order.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import lib as L
+
+_OP_N, _OP_T, _OP_C = 111, 112, 113
+_SIDE_R = 142
+_FILL_U = 121
+_EVECT = 211
+
+
+class _Dz(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class RocLinalg:
+    """A rocBLAS handle plus the four batched routines the setup needs."""
+
+    _inst = None
+
+    @classmethod
+    def get(cls):
+        if cls._inst is None:
+            cls._inst = cls()
+        return cls._inst
+
+    def __init__(self):
+        self.rb = ctypes.CDLL("librocblas.so")
+        self.rs = ctypes.CDLL("librocsolver.so")
+        h = ctypes.c_void_p()
+        st = self.rb.rocblas_create_handle(ctypes.byref(h))
+        if st != 0:
+            raise L.HipError(f"rocblas_create_handle failed ({st})")
+        self.h = h
+        I, P, S = ctypes.c_int, ctypes.c_void_p, ctypes.c_int64
+        self.rb.rocblas_zgemm_strided_batched.argtypes = [P, I, I, I, I, I, P, P, I, S, P, I, S, P, P, I, S, I]
+        self.rb.rocblas_zdgmm_strided_batched.argtypes = [P, I, I, I, P, I, S, P, I, S, P, I, S, I]
+        # the QR-iteration solver: the divide-and-conquer one returned non-finite eigenvectors (with info = 0) for the
+        # degenerate spectra of the level Grams at N >= 128
+        self.rs.rocsolver_zheev_strided_batched.argtypes = [P, I, I, I, P, I, S, P, S, P, S, P, I]
+        self.one, self.zero = _Dz(1.0, 0.0), _Dz(0.0, 0.0)
+
+    def gemm(self, opA, opB, N, A, B, C, batch):
+        """C_b = op(A_b) op(B_b) for N x N column-major matrices, batch-strided by N^2."""
+        s = N * N
+        st = self.rb.rocblas_zgemm_strided_batched(self.h, opA, opB, N, N, N, ctypes.byref(self.one), A, N, s, B, N, s,
+                                                   ctypes.byref(self.zero), C, N, s, batch)
+        if st != 0:
+            raise L.HipError(f"rocblas_zgemm_strided_batched failed ({st})")
+
+    def scale_cols(self, N, A, x, C, batch):
+        """C_b = A_b diag(x_b)."""
+        st = self.rb.rocblas_zdgmm_strided_batched(self.h, _SIDE_R, N, N, A, N, N * N, x, 1, N, C, N, N * N, batch)
+        if st != 0:
+            raise L.HipError(f"rocblas_zdgmm_strided_batched failed ({st})")
+
+    def eigh(self, N, A, D, E, info, batch):
+        """In place: A_b <- eigenvectors, D_b <- ascending eigenvalues of the Hermitian A_b."""
+        st = self.rs.rocsolver_zheev_strided_batched(self.h, _EVECT, _FILL_U, N, A, N, N * N, D, N, E, N, info, batch)
+        if st != 0:
+            raise L.HipError(f"rocsolver_zheev_strided_batched failed ({st})")
+
+
+def _level_device(n, z, ks, lo, hi):
+    """Column-major A_k over levels lo..hi for the frequencies ks, on the device (K, N, N)."""
+    N = 1 << n
+    X = np.tile(np.eye(N, dtype=complex), (len(ks), 1))
+    zr = np.ascontiguousarray(np.repeat(np.asarray(z)[ks], N))
+    L.check(L.load().adrt_bfly_rows(n, X.shape[0], L.ptr(zr), L.ptr(X), lo, hi, 0), "bfly_rows")
+    return L.DeviceArray.from_host(X.reshape(len(ks), N, N))
+
+
+def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
+    """Device arrays (K, N, N) of F_k, W_op,k and W_back,k in column-major order for the dense frequencies ks."""
+    N = 1 << n
+    ks = np.asarray(ks, dtype=np.int64)
+    K = ks.size
+    R = RocLinalg.get()
+    lo = 2 if kind == "hybrid" else 1
+    F = L.DeviceArray((K, N, N), np.complex128)
+    Wop = L.DeviceArray((K, N, N), np.complex128)
+    Wback = L.DeviceArray((K, N, N), np.complex128)
+    per = max(1, int(chunk_bytes // (7 * N * N * 16)))
+    nbytes = N * N * 16
+    for i0 in range(0, K, per):
+        kk = ks[i0 : i0 + per]
+        b = kk.size
+        off = i0 * nbytes
+        Ap = _level_device(n, z, kk, lo, n)
+        M = L.DeviceArray((b, N, N), np.complex128)
+        T1 = L.DeviceArray((b, N, N), np.complex128)
+        T2 = L.DeviceArray((b, N, N), np.complex128)
+        R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M = A^H A
+        D = L.DeviceArray((b, N), np.float64)
+        E = L.DeviceArray((b, N), np.float64)
+        info = L.DeviceArray((b,), np.int32)
+        R.eigh(N, M.ptr, D.ptr, E.ptr, info.ptr, b)  # M <- V
+        L.check(L.load().adrt_device_sync(), "sync")
+        if np.any(info.download() != 0):
+            raise L.HipError("rocsolver_zheev did not converge")
+        lam = np.maximum(D.download(), 0.0)
+        dinv = L.DeviceArray.from_host((1.0 / (lam + tau)).astype(np.complex128))
+        R.scale_cols(N, M.ptr, dinv.ptr, T1.ptr, b)  # V diag(1/(lam + tau))
+        Fc = F.ptr + off
+        R.gemm(_OP_N, _OP_C, N, T1.ptr, M.ptr, Fc, b)  # F = V D V^H
+        L.check(L.load().adrt_device_sync(), "sync")
+        chk = np.empty((b, N, N), np.complex128)
+        L.check(L.load().adrt_memcpy(chk.ctypes.data, Fc, chk.nbytes, 2, None), "d2h")
+        if not np.isfinite(chk).all():
+            raise L.HipError("non-finite dense filter from the GPU eigensolver")
+        R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M again
+        if kind == "hybrid":
+            B1 = _level_device(n, z, kk, 1, 1)
+            R.gemm(_OP_N, _OP_N, N, M.ptr, Fc, T1.ptr, b)  # M' F
+            R.gemm(_OP_N, _OP_N, N, T1.ptr, B1.ptr, T2.ptr, b)  # M' F B1
+            R.gemm(_OP_C, _OP_N, N, B1.ptr, T2.ptr, Wop.ptr + off, b)  # B1^H M' F B1
+            R.gemm(_OP_N, _OP_C, N, Fc, Ap.ptr, T1.ptr, b)  # F A'^H
+            R.gemm(_OP_C, _OP_N, N, B1.ptr, T1.ptr, Wback.ptr + off, b)  # B1^H F A'^H
+        else:
+            R.gemm(_OP_N, _OP_N, N, M.ptr, Fc, Wop.ptr + off, b)  # M F
+            R.gemm(_OP_N, _OP_C, N, Fc, Ap.ptr, Wback.ptr + off, b)  # F A^H
+        L.check(L.load().adrt_device_sync(), "sync")
+    return F, Wop, Wback
+
+
+def to_row_major(dev):
+    """Host row-major copy of device column-major matrices (K, N, N)."""
+    return np.ascontiguousarray(dev.download().transpose(0, 2, 1))

@@ -19,6 +19,7 @@ import ctypes
 import numpy as np
 
 from . import engine as E
+from . import gpusetup as G
 from . import lib as L
 from . import spectral
 
@@ -41,20 +42,15 @@ class _PlanBase:
         self.eng = E.HipEngine(self.n, self.z, self.deg, self.lam_max, self.tau, self.threads, copies=2,
                                dense_cap=dense_cap, p_levels=p_levels, precision=self.precision)
 
-    # dense rows run as a batched GEMM with precomputed pass matrices when these fit this many bytes
-    DENSE_GEMM_BYTES = 8 << 30
-
     def _make_plan(self, lo=1.0, hi=2.0, D=1):
         e = self.eng
         self._keep = [np.ascontiguousarray(e.z), np.ascontiguousarray(e.deg, dtype=np.int32), e.a, e.c]
-        Wop = Wback = None
+        Fd = Wop = Wback = None
         if e.n_dense:
-            self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), np.ascontiguousarray(e.dense_F)]
-            if 2 * e.n_dense * self.N * self.N * 16 <= self.DENSE_GEMM_BYTES:
-                kind = "hybrid" if self.KIND == L.KIND_HYBRID else "lp"
-                Wop, Wback = E.pass_matrices(self.n, self.z, e.dense_k, e.dense_F, kind)
-                Wop, Wback = np.ascontiguousarray(Wop), np.ascontiguousarray(Wback)
-                self._keep += [Wop, Wback]
+            # dense tables and pass matrices built on the GPU (rocSOLVER eigh + rocBLAS GEMMs), column-major on device
+            kind = "hybrid" if self.KIND == L.KIND_HYBRID else "lp"
+            Fd, Wop, Wback = G.dense_setup(self.n, self.z, e.dense_k, self.tau, kind)
+            self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), Fd, Wop, Wback]
         d = L.PlanDesc()
         d.kind = self.KIND
         d.n = self.n
@@ -69,15 +65,21 @@ class _PlanBase:
         d.z, d.deg, d.a, d.c = (L.ptr(a) for a in self._keep[:4])
         d.n_dense = e.n_dense
         d.dense_k = L.ptr(self._keep[4]) if e.n_dense else None
-        d.dense_F = L.ptr(self._keep[5]) if e.n_dense else None
+        d.dense_F = L.ptr(Fd)
         d.max_batch = int(self.max_batch)
-        d.dense_Wop = L.ptr(Wop) if Wop is not None else None
-        d.dense_Wback = L.ptr(Wback) if Wback is not None else None
+        d.dense_Wop = L.ptr(Wop)
+        d.dense_Wback = L.ptr(Wback)
         d.refine = int(self.refine)
+        d.dense_on_device = 1
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
         self._plan = h.value
+        if e.n_dense:  # the plan holds its own copies
+            del self._keep[5:]
+            Fd.free()
+            Wop.free()
+            Wback.free()
         N = self.N
         self._dimg = L.DeviceArray((self.max_batch, N, N), np.float64)
         self._dimg2 = L.DeviceArray((self.max_batch, N, N), np.float64)

@@ -118,6 +118,8 @@ typedef struct {
     const adrt_c128* dense_Wop;
     const adrt_c128* dense_Wback;
     int refine;         /* ADRT_PREC_F32 only: the backprojection and the last `refine` outer steps run in FP64 */
+    int dense_on_device; /* 1: dense_F / dense_Wop / dense_Wback are DEVICE pointers to column-major matrices
+                            (the row-major transposes), copied without a host round trip */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;

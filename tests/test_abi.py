@@ -46,6 +46,6 @@ def test_reference_signatures():
 
 def test_plan_desc_layout():
     """adrt_plan_desc field order and size as the C compiler lays it out
-    (6 ints, 3 doubles, 5 pointers, int + pad, 2 pointers, int + pad, 2 pointers, int + pad)."""
+    (6 ints, 3 doubles, 5 pointers, int + pad, 2 pointers, int + pad, 2 pointers, 2 ints)."""
     L = pkg("lib")
     assert ctypes.sizeof(L.PlanDesc) == 6 * 4 + 3 * 8 + 5 * 8 + 8 + 2 * 8 + 8 + 2 * 8 + 8

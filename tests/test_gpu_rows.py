@@ -94,3 +94,21 @@ def test_engine_fp32_close(gpu):
     eng64 = E.HipEngine(n, d["z"], d["deg"], d["lam_max"], float(d["tau"]), dense_cap=None)
     Z64 = eng64.run(d["Y"], OPS(n)["gh"])
     assert relmax(Z, Z64) < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["hybrid", "lp"])
+def test_gpu_dense_setup_matches_host(gpu, kind):
+    """Dense tables and pass matrices from rocSOLVER / rocBLAS equal the host eigh construction (fastops.py:28-39)."""
+    E = pkg("engine")
+    G = pkg("gpusetup")
+    n, N = 5, 32
+    z = np.exp(1j * 2 * np.pi * np.fft.fftfreq(2 * N))
+    ks = np.array([0, 1, 2, 62, 63], dtype=np.int32)
+    tau = 0.2 * N
+    lo = 2 if kind == "hybrid" else 1
+    F, Wop, Wback = G.dense_setup(n, z, ks, tau, kind)
+    Fh = E.dense_tables(n, z, tau, ks, lo, n)
+    Wo, Wb = E.pass_matrices(n, z, ks, Fh, kind)
+    assert relmax(G.to_row_major(F), Fh) < 1e-12
+    assert relmax(G.to_row_major(Wop), Wo) < 1e-12
+    assert relmax(G.to_row_major(Wback), Wb) < 1e-12
