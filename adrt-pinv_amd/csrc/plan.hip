@@ -508,6 +508,14 @@ int adrt_event_record(void* ev, void* stream) {
     return OK;
 }
 
+int adrt_event_query(void* ev) {
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    set_error(std::string("hipEventQuery failed: ") + hipGetErrorString(e));
+    return E_HIP;
+}
+
 int adrt_event_elapsed_ms(void* start, void* stop, float* ms) {
     ADRT_REQUIRE(ms, "event_elapsed: null pointer");
     ADRT_HIP(hipEventSynchronize((hipEvent_t)stop));

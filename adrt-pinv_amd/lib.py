@@ -44,6 +44,7 @@ SIGNATURES = {
     "adrt_event_destroy": (I, [P]),
     "adrt_event_record": (I, [P, P]),
     "adrt_event_elapsed_ms": (I, [P, P, ctypes.POINTER(ctypes.c_float)]),
+    "adrt_event_query": (I, [P]),
     "adrt_rows_config_set": (I, [I, I, I]),
     "adrt_rows_config_get": (I, [I, I]),
     "adrt_rows_plan_create": (I, [ctypes.POINTER(P), I, I, P, P, P, P, D, I, P, P, I]),
@@ -220,6 +221,12 @@ class Event:
 
     def record(self, stream=None):
         check(load().adrt_event_record(self.handle, stream.handle if stream else None), "event_record")
+
+    def done(self):
+        st = load().adrt_event_query(self.handle)
+        if st < 0:
+            check(st, "event_query")
+        return st == 1
 
     def elapsed_ms(self, stop):
         ms = ctypes.c_float()

@@ -69,6 +69,7 @@ int adrt_event_create(void** ev);
 int adrt_event_destroy(void* ev);
 int adrt_event_record(void* ev, void* stream);
 int adrt_event_elapsed_ms(void* start, void* stop, float* ms);
+int adrt_event_query(void* ev); /* 1 done, 0 pending, < 0 error */
 
 /* Row engine with device-resident parameters: the plug-in surface of fastops.RowEngine (fastops.py:20-167).
  * Row r of a run uses parameter row r % L.  Rows whose parameter row is listed in dense_k use the exact filter

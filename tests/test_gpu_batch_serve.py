@@ -1,0 +1,63 @@
+"""Mixed-N batches in one graph, the autotuner, and the serving loop on the GPU."""
+
+import numpy as np
+import pytest
+
+from conftest import pkg, relmax
+
+pytestmark = pytest.mark.gpu
+
+
+def _sinos(ns, seed0=300, noise=0.01):
+    ph = pkg("phantom")
+    R = pkg("radon")
+    return [ph.add_noise(R.adrt(ph.ellipses(seed0 + i, 1 << n)), noise, seed0 + 1000 + i) for i, n in enumerate(ns)]
+
+
+def test_mixed_batch_graph(gpu):
+    B = pkg("batch")
+    I = pkg("inverses")
+    C = pkg("plancache")
+    ns = [6, 7, 6, 8, 7, 6, 8]
+    b = _sinos(ns)
+    mb = B.MixedBatch(ns, precision="fp32", refine=2, dense_cap=12)
+    mb.set_inputs(b)
+    mb.run()
+    x1 = mb.outputs()
+    mb.run()  # a second replay of the same graph gives the same result
+    x2 = mb.outputs()
+    for i, n in enumerate(ns):
+        assert np.array_equal(x1[i], x2[i])
+        lam, outer = C.hybrid_setup(n, dense_cap=12)
+        H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=12)
+        ref = H.apply(b[i])
+        assert np.linalg.norm(x1[i] - ref) / np.linalg.norm(ref) < 1e-5, (i, n)
+        # the same FP32 plan on this sinogram alone gives the same result as in the batch
+        alone = mb.plans[n].apply(b[i])
+        assert relmax(x1[i], alone) < 1e-12
+    for n, (best, times) in mb.tuning.items():
+        assert best in times and times[best] == min(times.values())
+
+
+def test_server_results_same_alone_and_in_mix(gpu):
+    S = pkg("serve")
+    C = pkg("plancache")
+    keys = [(6, 8), (7, 8), (7, 16)]
+    setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
+    srv = S.Server(keys, slots_per_key=2, max_batch=3, setups=setups)
+    ns = [6, 7, 7, 6, 7, 6, 7, 7, 6, 7]
+    b = _sinos(ns, seed0=500)
+    reqs = [S.Request(i, n, 8 if i % 2 else 16 if n == 7 else 8, b[i], 0.0) for i, n in enumerate(ns)]
+    trace = [(0.0005 * i, r) for i, r in enumerate(reqs)]
+    bad = S.Request(99, 9, 8, np.zeros((4, 1023, 512)), 0.0)
+    done = srv.run_trace(sorted(trace + [(0.001, bad)], key=lambda t: t[0]))
+    assert bad.error and bad not in done
+    assert len(done) == len(reqs) and all(r.error is None for r in reqs)
+    # alone: each request through a fresh single-request call of the same slot type
+    for r in reqs:
+        slot = next(s for s in srv.slots if (s.n, s.cap) == (r.n, r.cap))
+        alone = slot.plan.apply(r.b)
+        assert relmax(r.x, alone) < 1e-12, r.rid
+    st = S.latency_stats(done)
+    assert st["count"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()

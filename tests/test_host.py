@@ -1,0 +1,144 @@
+"""CPU tests of the host logic: setup arithmetic, level-row counts, phantoms, plan cache, the serving scheduler
+(with a host stand-in for the GPU slots) and its failure handling.  No GPU call."""
+
+import numpy as np
+import pytest
+
+from conftest import golden, pkg
+from oracle import inverses as oiv
+
+
+def test_degrees_and_sym_match_oracle():
+    S = pkg("spectral")
+    d = golden("inverse_n6_phantom")
+    n = 6
+    lam, deg = S.degrees(d["h_lam_max"], 0.2 * (1 << n), 0.1)
+    assert np.array_equal(deg, d["h_deg"])
+    lam2, deg2 = oiv.degrees(d["h_lam_max"], 0.2 * (1 << n), 0.1)
+    assert np.array_equal(lam, lam2) and np.array_equal(deg, deg2)
+    oms = 2 * np.pi * np.fft.fftfreq(2 << n)
+    assert np.array_equal(S.explicit_lam_max(n, oms), oiv.explicit_bound(n, oms))
+
+
+def test_g1_host_stencil_matches_oracle():
+    S = pkg("spectral")
+    x = np.random.default_rng(0).standard_normal((16, 16))
+    assert np.array_equal(S.g1_apply(x), oiv.G1_apply(x))
+    assert np.array_equal(S.pi_apply(x), oiv.Pi_apply(x))
+
+
+@pytest.mark.parametrize("fixture", ["rows_n4_cap-1", "rows_n4_cap8", "rows_n5_cap8"])
+def test_level_step_counts_match_reference(fixture):
+    """The analytic level-row count equals the reference engine's count (fastops.py:137-154)."""
+    E = pkg("engine")
+    d = golden(fixture)
+    n, N = int(d["n"]), 1 << int(d["n"])
+    cap = int(d["dense_cap"])
+    L = len(d["z"])
+    ops = {"gh": [("B", 1, 1), ("P", 2, n), ("B", 2, n), ("BT", 2, n), ("BT", 1, 1)], "P_1n": [("P", 1, n)],
+           "B_1n": [("B", 1, n)], "lp_back": [("BT", 1, n), ("P", 1, n)]}
+    for p in (1, 2):
+        dense = np.nonzero(d["deg"] > cap)[0] if cap >= 0 else []
+        for name, o in ops.items():
+            assert E.level_steps(o, 2 * L, L, d["deg"], dense, n, N) == int(d[f"{name}_p{p}_count"]), name
+
+
+def test_phantom_and_noise():
+    ph = pkg("phantom")
+    a, b = ph.ellipses(3, 64), ph.ellipses(3, 64)
+    assert np.array_equal(a, b) and not np.array_equal(a, ph.ellipses(4, 64))
+    b0 = np.random.default_rng(0).standard_normal((4, 127, 64))
+    bn = ph.add_noise(b0, 0.01, 5)
+    assert abs(np.linalg.norm(bn - b0) / np.linalg.norm(b0) - 0.01) < 1e-12
+    x = ph.int_image(1, 32)
+    assert x.dtype == np.int32 and x.min() >= 0 and x.max() <= 255
+
+
+def test_plan_cache_roundtrip(tmp_path, monkeypatch):
+    monkeypatch.setenv("ADRT_PLAN_CACHE", str(tmp_path))
+    C = pkg("plancache")
+    assert C.load("hybrid", 5, dense_cap=8) is None
+    C.store("hybrid", 5, np.arange(3.0), (0.1, 0.9, 17), dense_cap=8)
+    hit = C.load("hybrid", 5, dense_cap=8)
+    assert np.array_equal(hit["lam_max"], np.arange(3.0)) and hit["outer"] == (0.1, 0.9, 17)
+    assert C.load("hybrid", 5, dense_cap=16) is None
+
+
+class HostSlot:
+    """Stand-in for a GPU slot: 'computes' x = sum of the sinogram per request after a fixed number of polls."""
+
+    def __init__(self, n, cap, max_batch, latency_polls=2):
+        self.n, self.cap, self.max_batch = n, cap, max_batch
+        self.busy, self.polls, self.latency = [], 0, latency_polls
+        self.batches = []
+
+    def device_bytes(self):
+        return 1000
+
+    def launch(self, reqs):
+        self.busy, self.polls = reqs, 0
+        self.batches.append([r.rid for r in reqs])
+
+    def poll(self):
+        if not self.busy:
+            return []
+        self.polls += 1
+        if self.polls < self.latency:
+            return []
+        done, self.busy = self.busy, []
+        for r in done:
+            r.x = np.full((1, 1), float(np.sum(r.b)))
+        return done
+
+    def close(self):
+        pass
+
+
+def test_server_continuous_batching_and_rejection():
+    S = pkg("serve")
+    made = []
+
+    def factory(n, cap, mb):
+        s = HostSlot(n, cap, mb)
+        made.append(s)
+        return s
+
+    srv = S.Server([(7, 16), (8, 16)], slots_per_key=2, max_batch=3, mem_budget=10**9, slot_factory=factory)
+    rng = np.random.default_rng(0)
+    reqs = [S.Request(i, 7 if i % 3 else 8, 16, rng.standard_normal(5), 0.0) for i in range(20)]
+    bad = S.Request(99, 11, 16, np.ones(3), 0.0)  # a size the server does not hold
+    for r in reqs[:10]:
+        srv.submit(r)
+    assert not srv.submit(bad) and bad.error
+    srv.step()
+    for r in reqs[10:]:  # arrivals while the first batches are in flight
+        srv.submit(r)
+    for _ in range(50):
+        srv.step()
+        if srv.idle():
+            break
+    assert srv.idle() and len(srv.done) == 20
+    for r in reqs:
+        assert r.x[0, 0] == np.sum(r.b) and r.error is None
+    assert all(len(b) <= 3 for s in made for b in s.batches)
+    assert sum(len(s.batches) for s in made) < 20  # requests were batched
+
+
+def test_server_memory_budget_limits_slots():
+    S = pkg("serve")
+    srv = S.Server([(7, 16), (8, 32)], slots_per_key=4, max_batch=2, mem_budget=3000,
+                   slot_factory=lambda n, cap, mb: HostSlot(n, cap, mb))
+    assert len(srv.slots) == 3 and srv.device_bytes <= 3000
+    assert srv.served[(7, 16)] and srv.served[(8, 32)]
+
+
+def test_poisson_trace_and_stats():
+    S = pkg("serve")
+    pool = {7: [np.zeros(2)], 8: [np.ones(2)]}
+    tr = S.poisson_trace(1000.0, 0.5, [7, 8], [0.5, 0.5], [16, 32], pool, seed=7)
+    gaps = np.diff([0.0] + [t for t, _ in tr])
+    assert 350 < len(tr) < 650 and abs(gaps.mean() - 1e-3) < 3e-4
+    for i, (_, r) in enumerate(tr):
+        r.t_arrive, r.t_done = 0.0, 0.001 * (i + 1)
+    st = S.latency_stats([r for _, r in tr])
+    assert st["count"] == len(tr) and st["p99_ms"] >= st["p50_ms"]
