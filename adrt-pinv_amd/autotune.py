@@ -54,7 +54,7 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, persist=True):
     """Time the row pass of ``plan`` (a FastLpHip / FastHybridHip with S <= max_batch sinograms) for every candidate
     mapping; install and return (best_loge, {loge: ms})."""
     lib = L.load()
-    prec = L.PRECISION[plan.precision]
+    prec = L.CONFIG_PRECISION[plan.precision]
     n = plan.n
     st = stream or L.Stream()
     dx = L.DeviceArray((S, plan.N, plan.N))

@@ -191,7 +191,7 @@ int cert_rowsum_impl(int n, int K, const double* omega, int m_lo, double* out) {
 
 // ---------------------------------------------------------------- device-resident row plan
 struct RowsPlan {
-    int n = 0, N = 0, L = 0, prec = PREC_F64;
+    int n = 0, N = 0, L = 0, prec = PREC_F64, comp = 0;
     double tau = 0;
     c128* z = nullptr;
     int* deg = nullptr;
@@ -335,12 +335,13 @@ int adrt_rows_plan_create(adrt_rows_plan** out, int n, int L, const adrt_c128* z
     ADRT_REQUIRE(z && deg && a && c, "rows_plan_create: null parameter array");
     ADRT_REQUIRE(n_dense >= 0 && n_dense <= L, "rows_plan_create: bad n_dense");
     ADRT_REQUIRE(n_dense == 0 || (dense_k && dense_F), "rows_plan_create: dense rows need dense_k and dense_F");
-    ADRT_REQUIRE(precision == PREC_F64 || precision == PREC_F32, "rows_plan_create: precision must be 0 or 1");
+    ADRT_REQUIRE(precision >= 0 && precision <= 2, "rows_plan_create: precision must be 0, 1 or 2");
     auto* p = new RowsPlan();
     p->n = n;
     p->N = 1 << n;
     p->L = L;
-    p->prec = precision;
+    p->prec = precision == PREC_F64 ? PREC_F64 : PREC_F32;
+    p->comp = precision == 1;
     p->tau = tau;
     p->n_dense = n_dense;
     auto fail = [&](int st) {
@@ -426,6 +427,7 @@ int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows
     A.dFT = p->dFT;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = RowOp{ops[3 * i], ops[3 * i + 1], ops[3 * i + 2]};
+    A.comp = p->comp;
     return launch_rows(A, p->prec, (hipStream_t)stream);
 }
 

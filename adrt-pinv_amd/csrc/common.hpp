@@ -34,6 +34,20 @@ __host__ __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b) { return {a.x * b
 template <typename T, typename U>
 __host__ __device__ inline cplx<T> cast(cplx<U> a) { return {(T)a.x, (T)a.y}; }
 
+// Error-free sum (Knuth's TwoSum): s = fl(a + b) and e with a + b = s + e exactly in round-to-nearest.  Additions
+// only, so contraction into FMAs cannot alter it.
+template <typename T>
+__host__ __device__ inline void two_sum(T a, T b, T& s, T& e) {
+    s = a + b;
+    const T bv = s - a;
+    e = (a - (s - bv)) + (b - bv);
+}
+template <typename T>
+__host__ __device__ inline void two_sum(cplx<T> a, cplx<T> b, cplx<T>& s, cplx<T>& e) {
+    two_sum(a.x, b.x, s.x, e.x);
+    two_sum(a.y, b.y, s.y, e.y);
+}
+
 // ---------------------------------------------------------------- error reporting (thread-local last error)
 void set_error(const std::string& msg);
 const char* last_error_cstr();
@@ -87,8 +101,8 @@ struct RowOp {
 };
 
 // Precision of the row engine.  F64: every row op in double.  F32: butterflies, Chebyshev steps and the
-// offset FFTs in float (compensated accumulation of the Chebyshev iterate), level 1, the dense table and
-// everything in image space in double.
+// offset FFTs in float; the dense table and everything in image space in double.  With compensation (RowsArgs::comp)
+// the FP32 Chebyshev iterate is carried as an unevaluated sum hi + lo, every increment added by TwoSum.
 enum Precision : int { PREC_F64 = 0, PREC_F32 = 1 };
 
 }  // namespace adrt

@@ -41,6 +41,9 @@ struct Plan {
     int nback = 0;
     RowOp ops_op[MAX_OPS];
     int nop = 0;
+    int comp = 0;  // compensated FP32 Chebyshev accumulation (desc precision ADRT_PREC_F32)
+    hipEvent_t ev_rows0 = nullptr, ev_rows1 = nullptr;  // bracket every row-kernel launch when set (timing)
+    bool borrowed = false;  // dFT / WopT / WbackT belong to the caller (desc dense_on_device == 2)
 };
 
 namespace {
@@ -50,6 +53,7 @@ size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(
 
 void free_plan(Plan* p) {
     if (!p) return;
+    if (p->borrowed) p->dFT = p->WopT = p->WbackT = nullptr;
     void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -78,19 +82,27 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
     A.group_by_k = 1;
     A.SP = 2 * S;
     A.korder = p->korder;
+    A.comp = prec == PREC_F32 ? p->comp : 0;
     if (WT && p->d.n_dense > 0) {
         // Chebyshev rows in the row kernel, dense rows as one GEMM with the pass matrix
         A.korder = p->korder_nd;
         A.nk = p->L - p->d.n_dense;
         A.skip_dense = 1;
-        if (A.nk > 0) ADRT_TRY(launch_rows(A, prec, st));
+        if (A.nk > 0) {
+            if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
+            ADRT_TRY(launch_rows(A, prec, st));
+            if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
+        }
         if (p->d.n >= 4 && 2 * S > 4) {  // the MFMA tile (64 rows) pays off once several sinograms share W
             if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
             return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, p->Y2, prec, st);
         }
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
     }
-    return launch_rows(A, prec, st);
+    if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
+    ADRT_TRY(launch_rows(A, prec, st));
+    if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
+    return OK;
 }
 
 // One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
@@ -152,7 +164,8 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
     const size_t N = (size_t)1 << d->n, L = 2 * N, S = d->max_batch > 0 ? d->max_batch : 1;
     size_t b = 0;
     b += L * (sizeof(c128) + sizeof(int) + 2 * sizeof(double) + sizeof(int));
-    b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
+    if (d->dense_on_device != 2)
+        b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
     b += twiddle_bytes(d->n, PREC_F64) + twiddle_bytes(d->n, PREC_F32);
     b += S * 2 * L * N * row_elem(PREC_F64);
     b += S * 4 * N * N * strip_elem(PREC_F64);
@@ -165,18 +178,22 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     *out = nullptr;
     ADRT_REQUIRE(d->kind == ADRT_KIND_LP || d->kind == ADRT_KIND_HYBRID, "plan_create: kind must be LP or HYBRID");
     ADRT_REQUIRE(d->n >= 1 && d->n <= 11, "plan_create: need 1 <= n <= 11");
-    ADRT_REQUIRE(d->precision == PREC_F64 || d->precision == PREC_F32, "plan_create: precision must be 0 or 1");
+    ADRT_REQUIRE(d->precision >= ADRT_PREC_F64 && d->precision <= ADRT_PREC_F32_PLAIN,
+                 "plan_create: precision must be 0, 1 or 2");
     ADRT_REQUIRE(d->precond >= 0 && d->precond <= 2, "plan_create: precond must be 0, 1 or 2");
     ADRT_REQUIRE(d->g1_deg >= 1, "plan_create: g1_deg must be >= 1");
     ADRT_REQUIRE(d->outer_deg >= 1, "plan_create: outer_deg must be >= 1");
     ADRT_REQUIRE(d->hi > 0 && d->lo > -d->hi, "plan_create: need hi > 0 and hi + lo > 0");
     ADRT_REQUIRE(d->max_batch >= 1, "plan_create: max_batch must be >= 1");
     ADRT_REQUIRE(d->refine >= 0, "plan_create: refine must be >= 0");
+    ADRT_REQUIRE(d->dense_on_device >= 0 && d->dense_on_device <= 2, "plan_create: dense_on_device must be 0, 1 or 2");
     ADRT_REQUIRE(d->z && d->deg && d->a && d->c, "plan_create: null row parameter array");
     ADRT_REQUIRE(d->n_dense >= 0 && d->n_dense <= (2 << d->n), "plan_create: bad n_dense");
     ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && d->dense_F), "plan_create: dense rows need dense_k and dense_F");
     auto* p = new Plan();
     p->d = *d;
+    p->comp = d->precision == ADRT_PREC_F32;
+    if (d->precision == ADRT_PREC_F32_PLAIN) p->d.precision = PREC_F32;
     p->d.lam_max = nullptr;
     p->d.z = nullptr;
     p->d.deg = nullptr;
@@ -216,13 +233,18 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
         if ((st = alloc((void**)&p->dslot, sizeof(int) * L)) || (st = up(p->dslot, slot.data(), sizeof(int) * L)))
             return fail(st);
         const size_t tb = sizeof(c128) * (size_t)d->n_dense * N * N;
-        if ((st = alloc((void**)&p->dFT, tb))) return fail(st);
-        if (d->dense_on_device) {
+        if (d->dense_on_device == 2) {
+            p->borrowed = true;
+            p->dFT = (c128*)d->dense_F;
+        } else if ((st = alloc((void**)&p->dFT, tb))) {
+            return fail(st);
+        }
+        if (d->dense_on_device == 1) {
             if (hipMemcpy(p->dFT, d->dense_F, tb, hipMemcpyDeviceToDevice) != hipSuccess) {
                 set_error("plan_create: device copy of the dense tables failed");
                 return fail(E_HIP);
             }
-        } else {
+        } else if (d->dense_on_device == 0) {
             std::vector<c128> ft((size_t)d->n_dense * N * N);
             const c128* F = (const c128*)d->dense_F;
             for (int s = 0; s < d->n_dense; ++s)
@@ -237,6 +259,10 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
         const size_t nn = (size_t)N * N;
         std::vector<c128> t(d->dense_on_device ? 0 : (size_t)d->n_dense * nn);
         auto transpose_up = [&](const adrt_c128* W, c128** dst) -> int {
+            if (d->dense_on_device == 2) {
+                *dst = (c128*)W;
+                return OK;
+            }
             if (d->dense_on_device) {
                 int r = alloc((void**)dst, sizeof(c128) * d->n_dense * nn);
                 if (r) return r;
@@ -377,6 +403,15 @@ int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream) {
     if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
     return which == 0 ? rows_pass(p, p->d.precision, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2, p->WbackT)
                       : rows_pass(p, p->d.precision, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2, p->WopT);
+}
+
+int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end) {
+    auto* p = (Plan*)plan;
+    ADRT_REQUIRE(p, "plan_time_rows: null plan");
+    ADRT_REQUIRE((ev_begin == nullptr) == (ev_end == nullptr), "plan_time_rows: give both events or neither");
+    p->ev_rows0 = (hipEvent_t)ev_begin;
+    p->ev_rows1 = (hipEvent_t)ev_end;
+    return OK;
 }
 
 // x = outer Chebyshev of degree D on P G with right-hand side y = backproject(b) (fastops.py:247-258, 287-288, 337-338)

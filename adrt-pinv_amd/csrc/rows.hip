@@ -186,10 +186,15 @@ __global__ __launch_bounds__(NT) void rows_kernel(RowsArgs A) {
                     const double ak = A.a[k], ck = A.c[k];
                     const int d = A.deg[k];
                     const T tau = (T)A.tau;
+                    // compensated FP32: the iterate is zi + zl, increments added by TwoSum
+                    const bool comp = A.comp && sizeof(T) == sizeof(float);
+                    cplx<T> zl[E], zpl[E];
 #pragma unroll
                     for (int j = 0; j < E; ++j) {
                         zi[j] = scale((T)ak, y[j]);
                         zp[j] = mk((T)0, (T)0);
+                        zl[j] = comp ? mk(fma((T)ak, y[j].x, -zi[j].x), fma((T)ak, y[j].y, -zi[j].y)) : mk((T)0, (T)0);
+                        zpl[j] = mk((T)0, (T)0);
                     }
                     double om = 1.0;
                     for (int it = 1; it < d; ++it) {
@@ -207,7 +212,16 @@ __global__ __launch_bounds__(NT) void rows_kernel(RowsArgs A) {
 #pragma unroll
                         for (int j = 0; j < E; ++j) {
                             const int p = tid + NT * j;
-                            if (p < N) {
+                            if (p < N && comp) {
+                                const cplx<T> r = sub(sub(sub(y[j], X[p]), scale(tau, zi[j])), scale(tau, zl[j]));
+                                const cplx<T> dz = add(sub(zi[j], zp[j]), sub(zl[j], zpl[j]));
+                                cplx<T> hi, err;
+                                two_sum(zi[j], add(scale(w1, scale(aT, r)), scale(w0, dz)), hi, err);
+                                zp[j] = zi[j];
+                                zpl[j] = zl[j];
+                                zi[j] = hi;
+                                zl[j] = add(zl[j], err);
+                            } else if (p < N) {
                                 const cplx<T> r = sub(sub(y[j], X[p]), scale(tau, zi[j]));
                                 const cplx<T> zn = add(add(zi[j], scale(w1, scale(aT, r))), scale(w0, sub(zi[j], zp[j])));
                                 zp[j] = zi[j];
@@ -220,7 +234,7 @@ __global__ __launch_bounds__(NT) void rows_kernel(RowsArgs A) {
 #pragma unroll
                     for (int j = 0; j < E; ++j) {
                         const int p = tid + NT * j;
-                        if (p < N) X[p] = zi[j];
+                        if (p < N) X[p] = comp ? add(zi[j], zl[j]) : zi[j];
                     }
                     __syncthreads();
                 }

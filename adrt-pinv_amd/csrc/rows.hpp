@@ -37,6 +37,7 @@ struct RowsArgs {
     int nk = 0;          // frequencies listed in korder (0 = period)
     int skip_dense = 0;  // rows of dense frequencies are left untouched (a dense GEMM handles them)
     int loge = 0;  // register kernel elements per lane (log2); 0 = default for (n, precision), -1 = generic kernel
+    int comp = 0;  // FP32 only: compensated (TwoSum) accumulation of the Chebyshev iterate
 };
 
 // Host launcher (rows.hip).  prec selects the compute type of the butterflies and Chebyshev steps; the global row

@@ -197,7 +197,7 @@ __device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cons
     __syncthreads();
 }
 
-template <typename T, int LOGN, int LOGE>
+template <typename T, int LOGN, int LOGE, bool COMP>
 __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegArgs RA) {
     using C = Cfg<T, LOGN, LOGE>;
     constexpr int E = C::E;
@@ -309,7 +309,9 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
                 anyd |= sdeg[r] & 1;
             }
             __syncthreads();
-            cplx<T> y[E], zi[E], zp[E];
+            // the iterate is zi (+ zl when COMP: the rounding errors of its increments, kept by TwoSum)
+            constexpr int EL = COMP ? E : 1;
+            cplx<T> y[E], zi[E], zp[E], zl[EL], zpl[EL];
             const double ak = valid ? A.a[k] : 0.0, ck = valid ? A.c[k] : 0.0;
             const T aT = (T)ak, tau = (T)A.tau;
 #pragma unroll
@@ -317,6 +319,10 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
                 y[u] = v[u];
                 zi[u] = scale(aT, y[u]);
                 zp[u] = mk((T)0, (T)0);
+                if constexpr (COMP) {
+                    zl[u] = mk(fma(aT, y[u].x, -zi[u].x), fma(aT, y[u].y, -zi[u].y));
+                    zpl[u] = mk((T)0, (T)0);
+                }
             }
             double om = 1.0;
             for (int it = 1; it < dmax; ++it) {
@@ -328,10 +334,21 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
                     const T w1 = (T)omj, w0 = (T)(omj - 1.0);
 #pragma unroll
                     for (int u = 0; u < E; ++u) {
-                        const cplx<T> r = sub(sub(y[u], v[u]), scale(tau, zi[u]));
-                        const cplx<T> zn = add(add(zi[u], scale(w1, scale(aT, r))), scale(w0, sub(zi[u], zp[u])));
-                        zp[u] = zi[u];
-                        zi[u] = zn;
+                        if constexpr (COMP) {
+                            const cplx<T> r = sub(sub(sub(y[u], v[u]), scale(tau, zi[u])), scale(tau, zl[u]));
+                            const cplx<T> dz = add(sub(zi[u], zp[u]), sub(zl[u], zpl[u]));
+                            cplx<T> hi, err;
+                            two_sum(zi[u], add(scale(w1, scale(aT, r)), scale(w0, dz)), hi, err);
+                            zp[u] = zi[u];
+                            zpl[u] = zl[u];
+                            zi[u] = hi;
+                            zl[u] = add(zl[u], err);
+                        } else {
+                            const cplx<T> r = sub(sub(y[u], v[u]), scale(tau, zi[u]));
+                            const cplx<T> zn = add(add(zi[u], scale(w1, scale(aT, r))), scale(w0, sub(zi[u], zp[u])));
+                            zp[u] = zi[u];
+                            zi[u] = zn;
+                        }
                     }
                     om = omj;
                 }
@@ -353,12 +370,20 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
                         for (int u = 0; u < E; ++u) acc[u] = add(acc[u], mul(Fq[u], yq));
                     }
 #pragma unroll
-                    for (int u = 0; u < E; ++u) zi[u] = cast<T>(acc[u]);
+                    for (int u = 0; u < E; ++u) {
+                        zi[u] = cast<T>(acc[u]);
+                        if constexpr (COMP) zl[u] = cast<T>(sub(acc[u], cast<double>(zi[u])));
+                    }
                 }
                 __syncthreads();
             }
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = zi[u];
+            for (int u = 0; u < E; ++u) {
+                if constexpr (COMP)
+                    v[u] = add(zi[u], zl[u]);
+                else
+                    v[u] = zi[u];
+            }
         }
     }
     if (valid) {
@@ -368,7 +393,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
     }
 }
 
-template <typename T, int LOGN, int LOGE>
+template <typename T, int LOGN, int LOGE, bool COMP>
 int launch_cfg(const RegArgs& RA, hipStream_t s) {
     using C = Cfg<T, LOGN, LOGE>;
     long long nblk;
@@ -381,35 +406,35 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         set_error("rows_reg: too many blocks");
         return E_INVALID;
     }
-    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RA);
+    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RA);
     return launch_check("rows_reg_kernel");
 }
 
-template <typename T, int LOGN>
+template <typename T, int LOGN, bool COMP>
 int launch_n(const RegArgs& RA, int loge, hipStream_t s) {
     if constexpr (LOGN >= 2) {
-        if (loge == 2) return launch_cfg<T, LOGN, 2>(RA, s);
+        if (loge == 2) return launch_cfg<T, LOGN, 2, COMP>(RA, s);
     }
     if constexpr (LOGN >= 3) {
-        if (loge == 3) return launch_cfg<T, LOGN, 3>(RA, s);
+        if (loge == 3) return launch_cfg<T, LOGN, 3, COMP>(RA, s);
     }
     if constexpr (LOGN >= 4) {
-        if (loge == 4) return launch_cfg<T, LOGN, 4>(RA, s);
+        if (loge == 4) return launch_cfg<T, LOGN, 4, COMP>(RA, s);
     }
     set_error("rows_reg: unsupported elements-per-lane");
     return E_INVALID;
 }
 
-template <typename T>
+template <typename T, bool COMP>
 int launch_t(const RegArgs& RA, int loge, hipStream_t s) {
     switch (RA.r.n) {
-        case 5: return launch_n<T, 5>(RA, loge, s);
-        case 6: return launch_n<T, 6>(RA, loge, s);
-        case 7: return launch_n<T, 7>(RA, loge, s);
-        case 8: return launch_n<T, 8>(RA, loge, s);
-        case 9: return launch_n<T, 9>(RA, loge, s);
-        case 10: return launch_n<T, 10>(RA, loge, s);
-        case 11: return launch_n<T, 11>(RA, loge, s);
+        case 5: return launch_n<T, 5, COMP>(RA, loge, s);
+        case 6: return launch_n<T, 6, COMP>(RA, loge, s);
+        case 7: return launch_n<T, 7, COMP>(RA, loge, s);
+        case 8: return launch_n<T, 8, COMP>(RA, loge, s);
+        case 9: return launch_n<T, 9, COMP>(RA, loge, s);
+        case 10: return launch_n<T, 10, COMP>(RA, loge, s);
+        case 11: return launch_n<T, 11, COMP>(RA, loge, s);
         default: set_error("rows_reg: n must be 5..11"); return E_INVALID;
     }
 }
@@ -463,8 +488,9 @@ int launch_rows_reg(const RowsArgs& A, int prec, int loge, hipStream_t s) {
     RA.korder = A.korder;
     RA.nk = A.nk > 0 ? A.nk : A.period;
     if (A.skip_dense && !A.group_by_k) return E_INVALID;  // only the frequency-grouped mapping can skip dense rows
-    if (prec == PREC_F64) return launch_t<double>(RA, loge, s);
-    return launch_t<float>(RA, loge, s);
+    if (prec == PREC_F64) return launch_t<double, false>(RA, loge, s);
+    if (A.comp) return launch_t<float, true>(RA, loge, s);
+    return launch_t<float, false>(RA, loge, s);
 }
 
 }  // namespace adrt

@@ -85,8 +85,8 @@ def dense_tables(n, z, tau, ks, lo, hi):
 class HipEngine:
     """GPU row engine with the RowEngine constructor and ``run`` (fastops.py:20-167).
 
-    ``threads`` is accepted and ignored (the GPU runs all rows at once).  ``precision`` ('fp64' | 'fp32') selects the
-    row arithmetic; 'fp64' is the drop-in default."""
+    ``threads`` is accepted and ignored (the GPU runs all rows at once).  ``precision`` ('fp64' | 'fp32' | 'fp32_plain')
+    selects the row arithmetic; 'fp64' is the drop-in default."""
 
     def __init__(self, n, z, deg, lam_max, tau, threads=1, copies=2, dense_cap=None, p_levels=(1, None),
                  precision="fp64"):

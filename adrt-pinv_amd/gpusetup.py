@@ -1,12 +1,18 @@
-"""Dense low-frequency tables built on the GPU (setup only): rocSOLVER batched Hermitian eigensolves (QR iteration)
-and rocBLAS batched complex GEMMs, reached through ctypes.
+"""Dense low-frequency tables built on the GPU (setup only): rocSOLVER batched Cholesky solves and rocBLAS batched
+complex GEMMs, reached through ctypes.
 
 For every dense frequency k (degree above dense_cap) the reference builds F_k = (M_k + tau)^{-1} with
-M_k = A_k^H A_k over the P levels, by eigh with the eigenvalues clipped at 0 (fastops.py:28-39).  The device plan also
-wants the whole op list of such a row as one matrix (dense.hip): for the hybrid G_h = B1^H M' F B1 and the
-backprojection B1^H F A'^H; for L_p M F and F A^H.  All matrices live on the device in column-major order, which is
-the row-major transpose the plan stores.  The level matrices come from the package's own butterflies applied to the
-identity: the rows it returns are the columns of A, i.e. A in column-major order.
+M_k = A_k^H A_k over the P levels, by eigh with the eigenvalues clipped at 0 (fastops.py:28-39).  Here F_k comes from
+the Cholesky factor of M_k + tau I and two triangular solves against the identity.  Both give (M_k + tau)^{-1}: the
+clip only lifts round-off eigenvalues of the PSD Gram (|lambda| ~ 1e-13 lambda_max) to 0, which moves F by far less
+than its condition number (1 + lambda_max / tau <= 1 + 5N) times the unit round-off.  The device plan also wants the
+whole op list of such a row as one matrix (dense.hip): for the hybrid G_h = B1^H M' F B1 and the backprojection
+B1^H F A'^H; for L_p M F and F A^H.  All matrices live on the device in column-major order, which is the row-major
+transpose the plan stores.  The level matrices come from the package's own butterflies applied to the identity: the
+rows it returns are the columns of A, i.e. A in column-major order.
+
+Tables are cached per (kind, n, tau, dense frequencies) for the life of the process (``clear_tables`` drops the
+cache), and plans borrow them, so every plan of one size shares one copy in HBM.
 """
 
 import ctypes
@@ -16,9 +22,7 @@ import numpy as np
 from . import lib as L
 
 _OP_N, _OP_T, _OP_C = 111, 112, 113
-_SIDE_R = 142
 _FILL_U = 121
-_EVECT = 211
 
 
 class _Dz(ctypes.Structure):
@@ -26,7 +30,7 @@ class _Dz(ctypes.Structure):
 
 
 class RocLinalg:
-    """A rocBLAS handle plus the four batched routines the setup needs."""
+    """A rocBLAS handle plus the batched routines the setup needs."""
 
     _inst = None
 
@@ -46,11 +50,11 @@ class RocLinalg:
         self.h = h
         I, P, S = ctypes.c_int, ctypes.c_void_p, ctypes.c_int64
         self.rb.rocblas_zgemm_strided_batched.argtypes = [P, I, I, I, I, I, P, P, I, S, P, I, S, P, P, I, S, I]
-        self.rb.rocblas_zdgmm_strided_batched.argtypes = [P, I, I, I, P, I, S, P, I, S, P, I, S, I]
-        # the QR-iteration solver: the divide-and-conquer one returned non-finite eigenvectors (with info = 0) for the
-        # degenerate spectra of the level Grams at N >= 128
-        self.rs.rocsolver_zheev_strided_batched.argtypes = [P, I, I, I, P, I, S, P, S, P, S, P, I]
+        self.rb.rocblas_zaxpy_strided_batched.argtypes = [P, I, P, P, I, S, P, I, S, I]
+        self.rs.rocsolver_zpotrf_strided_batched.argtypes = [P, I, I, P, I, S, P, I]
+        self.rs.rocsolver_zpotrs_strided_batched.argtypes = [P, I, I, I, P, I, S, P, I, S, I]
         self.one, self.zero = _Dz(1.0, 0.0), _Dz(0.0, 0.0)
+        self._ones = {}
 
     def gemm(self, opA, opB, N, A, B, C, batch):
         """C_b = op(A_b) op(B_b) for N x N column-major matrices, batch-strided by N^2."""
@@ -60,17 +64,27 @@ class RocLinalg:
         if st != 0:
             raise L.HipError(f"rocblas_zgemm_strided_batched failed ({st})")
 
-    def scale_cols(self, N, A, x, C, batch):
-        """C_b = A_b diag(x_b)."""
-        st = self.rb.rocblas_zdgmm_strided_batched(self.h, _SIDE_R, N, N, A, N, N * N, x, 1, N, C, N, N * N, batch)
+    def add_diag(self, N, A, alpha, batch):
+        """A_b += alpha I."""
+        if N not in self._ones:
+            self._ones[N] = L.DeviceArray.from_host(np.ones(N, dtype=np.complex128))
+        al = _Dz(float(alpha), 0.0)
+        st = self.rb.rocblas_zaxpy_strided_batched(self.h, N, ctypes.byref(al), self._ones[N].ptr, 1, 0, A, N + 1,
+                                                   N * N, batch)
         if st != 0:
-            raise L.HipError(f"rocblas_zdgmm_strided_batched failed ({st})")
+            raise L.HipError(f"rocblas_zaxpy_strided_batched failed ({st})")
 
-    def eigh(self, N, A, D, E, info, batch):
-        """In place: A_b <- eigenvectors, D_b <- ascending eigenvalues of the Hermitian A_b."""
-        st = self.rs.rocsolver_zheev_strided_batched(self.h, _EVECT, _FILL_U, N, A, N, N * N, D, N, E, N, info, batch)
+    def potrf(self, N, A, info, batch):
+        """In place: the upper Cholesky factor of the Hermitian positive definite A_b."""
+        st = self.rs.rocsolver_zpotrf_strided_batched(self.h, _FILL_U, N, A, N, N * N, info, batch)
         if st != 0:
-            raise L.HipError(f"rocsolver_zheev_strided_batched failed ({st})")
+            raise L.HipError(f"rocsolver_zpotrf_strided_batched failed ({st})")
+
+    def potrs(self, N, A, B, batch):
+        """B_b <- (U_b^H U_b)^{-1} B_b with the factor from potrf."""
+        st = self.rs.rocsolver_zpotrs_strided_batched(self.h, _FILL_U, N, N, A, N, N * N, B, N, N * N, batch)
+        if st != 0:
+            raise L.HipError(f"rocsolver_zpotrs_strided_batched failed ({st})")
 
 
 def _level_device(n, z, ks, lo, hi):
@@ -80,6 +94,26 @@ def _level_device(n, z, ks, lo, hi):
     zr = np.ascontiguousarray(np.repeat(np.asarray(z)[ks], N))
     L.check(L.load().adrt_bfly_rows(n, X.shape[0], L.ptr(zr), L.ptr(X), lo, hi, 0), "bfly_rows")
     return L.DeviceArray.from_host(X.reshape(len(ks), N, N))
+
+
+_TABLES = {}
+
+
+def clear_tables():
+    """Drops the cached tables (plans that borrow them keep their own references)."""
+    _TABLES.clear()
+
+
+def dense_tables(n, z, ks, tau, kind):
+    """(F, W_op, W_back) for the dense frequencies ks, from the cache or built by ``dense_setup``."""
+    key = (kind, int(n), float(tau), tuple(int(k) for k in ks))
+    if key not in _TABLES:
+        _TABLES[key] = dense_setup(n, z, ks, tau, kind)
+    return _TABLES[key]
+
+
+def table_bytes(tables):
+    return sum(t.nbytes for t in tables)
 
 
 def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
@@ -94,6 +128,7 @@ def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
     Wback = L.DeviceArray((K, N, N), np.complex128)
     per = max(1, int(chunk_bytes // (7 * N * N * 16)))
     nbytes = N * N * 16
+    lib = L.load()
     for i0 in range(0, K, per):
         kk = ks[i0 : i0 + per]
         b = kk.size
@@ -102,25 +137,22 @@ def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
         M = L.DeviceArray((b, N, N), np.complex128)
         T1 = L.DeviceArray((b, N, N), np.complex128)
         T2 = L.DeviceArray((b, N, N), np.complex128)
-        R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M = A^H A
-        D = L.DeviceArray((b, N), np.float64)
-        E = L.DeviceArray((b, N), np.float64)
         info = L.DeviceArray((b,), np.int32)
-        R.eigh(N, M.ptr, D.ptr, E.ptr, info.ptr, b)  # M <- V
-        L.check(L.load().adrt_device_sync(), "sync")
-        if np.any(info.download() != 0):
-            raise L.HipError("rocsolver_zheev did not converge")
-        lam = np.maximum(D.download(), 0.0)
-        dinv = L.DeviceArray.from_host((1.0 / (lam + tau)).astype(np.complex128))
-        R.scale_cols(N, M.ptr, dinv.ptr, T1.ptr, b)  # V diag(1/(lam + tau))
+        R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M = A^H A
+        L.check(lib.adrt_memcpy(T2.ptr, M.ptr, M.nbytes, 3, None), "d2d")
+        R.add_diag(N, T2.ptr, tau, b)  # M + tau I
+        R.potrf(N, T2.ptr, info.ptr, b)
         Fc = F.ptr + off
-        R.gemm(_OP_N, _OP_C, N, T1.ptr, M.ptr, Fc, b)  # F = V D V^H
-        L.check(L.load().adrt_device_sync(), "sync")
+        L.check(lib.adrt_memset(Fc, 0, b * nbytes, None), "memset")
+        R.add_diag(N, Fc, 1.0, b)  # I
+        R.potrs(N, T2.ptr, Fc, b)  # F = (M + tau I)^{-1}
+        L.check(lib.adrt_device_sync(), "sync")
+        if np.any(info.download() != 0):
+            raise L.HipError("rocsolver_zpotrf: shifted Gram not positive definite")
         chk = np.empty((b, N, N), np.complex128)
-        L.check(L.load().adrt_memcpy(chk.ctypes.data, Fc, chk.nbytes, 2, None), "d2h")
+        L.check(lib.adrt_memcpy(chk.ctypes.data, Fc, chk.nbytes, 2, None), "d2h")
         if not np.isfinite(chk).all():
-            raise L.HipError("non-finite dense filter from the GPU eigensolver")
-        R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M again
+            raise L.HipError("non-finite dense filter from the GPU solver")
         if kind == "hybrid":
             B1 = _level_device(n, z, kk, 1, 1)
             R.gemm(_OP_N, _OP_N, N, M.ptr, Fc, T1.ptr, b)  # M' F
@@ -131,7 +163,7 @@ def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
         else:
             R.gemm(_OP_N, _OP_N, N, M.ptr, Fc, Wop.ptr + off, b)  # M F
             R.gemm(_OP_N, _OP_C, N, Fc, Ap.ptr, Wback.ptr + off, b)  # F A^H
-        L.check(L.load().adrt_device_sync(), "sync")
+        L.check(lib.adrt_device_sync(), "sync")
     return F, Wop, Wback
 
 

@@ -7,8 +7,9 @@ FastHybrid (fastcpp.py:137-169, fastops.py:261-338): ``lam_max``, ``deg``, ``lo`
 Unlike the reference, the whole packed pass (strips, offset FFT, row ops, inverse FFT, quadrant sum) and the outer
 Chebyshev iteration run on the GPU through a device plan; only the setup (bounds, dense tables, the outer interval's
 eigensolve) touches the host.  Extra keywords:
-  precision  'fp64' (default, the plain path) or 'fp32' (float FFTs / butterflies / Chebyshev steps; dense table,
-             preconditioner and outer iteration stay in double)
+  precision  'fp64' (default, the plain path), 'fp32' (float FFTs / butterflies / Chebyshev steps with the Chebyshev
+             iterate accumulated by TwoSum compensation; dense table, preconditioner and outer iteration stay in
+             double) or 'fp32_plain' ('fp32' without the compensation)
   max_batch  sinograms per device call (``apply_batch``)
   outer      (lo, hi, D) to skip the eigensolve (e.g. to reuse the reference's setup scalars)
   refine     with precision 'fp32': the backprojection and the last ``refine`` outer steps run in FP64
@@ -46,10 +47,13 @@ class _PlanBase:
         e = self.eng
         self._keep = [np.ascontiguousarray(e.z), np.ascontiguousarray(e.deg, dtype=np.int32), e.a, e.c]
         Fd = Wop = Wback = None
+        self.table_bytes = 0
         if e.n_dense:
-            # dense tables and pass matrices built on the GPU (rocSOLVER eigh + rocBLAS GEMMs), column-major on device
+            # dense tables and pass matrices built on the GPU (rocSOLVER Cholesky + rocBLAS GEMMs), column-major on
+            # the device, shared by every plan of this size (the plan borrows them; _keep holds them alive)
             kind = "hybrid" if self.KIND == L.KIND_HYBRID else "lp"
-            Fd, Wop, Wback = G.dense_setup(self.n, self.z, e.dense_k, self.tau, kind)
+            Fd, Wop, Wback = G.dense_tables(self.n, self.z, e.dense_k, self.tau, kind)
+            self.table_bytes = G.table_bytes((Fd, Wop, Wback))
             self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), Fd, Wop, Wback]
         d = L.PlanDesc()
         d.kind = self.KIND
@@ -70,16 +74,11 @@ class _PlanBase:
         d.dense_Wop = L.ptr(Wop)
         d.dense_Wback = L.ptr(Wback)
         d.refine = int(self.refine)
-        d.dense_on_device = 1
+        d.dense_on_device = 2
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
         self._plan = h.value
-        if e.n_dense:  # the plan holds its own copies
-            del self._keep[5:]
-            Fd.free()
-            Wop.free()
-            Wback.free()
         N = self.N
         self._dimg = L.DeviceArray((self.max_batch, N, N), np.float64)
         self._dimg2 = L.DeviceArray((self.max_batch, N, N), np.float64)
@@ -170,6 +169,7 @@ class _PlanBase:
         if getattr(self, "_plan", None):
             L.load().adrt_plan_destroy(self._plan)
             self._plan = None
+            self._keep = []  # releases this plan's hold on the shared dense tables
         if getattr(self, "eng", None):
             self.eng.close()
 

@@ -58,6 +58,9 @@ SIGNATURES = {
     "adrt_plan_operator": (I, [P, P, P, I, P]),
     "adrt_plan_precond": (I, [P, P, P, I, P]),
     "adrt_plan_rows_pass": (I, [P, I, I, P]),
+    "adrt_plan_time_rows": (I, [P, P, P]),
+    "adrt_cgls_work_bytes": (ULL, [I, I, I]),
+    "adrt_cgls": (I, [I, P, P, I, D, I, I, P, P, P, I, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),
     "adrt_graph_begin": (I, [P]),
     "adrt_graph_end": (I, [P, ctypes.POINTER(P)]),
@@ -84,7 +87,9 @@ class PlanDesc(ctypes.Structure):
 
 KIND_LP, KIND_HYBRID = 0, 1
 PRECOND = {"none": 0, "g1": 1, "lorr": 2}
-PRECISION = {"fp64": 0, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "fp32_plain": 2}
+# the row-kernel configuration table is kept per compute type: both FP32 variants share one entry
+CONFIG_PRECISION = {"fp64": 0, "fp32": 1, "fp32_plain": 1}
 OPS = {"B": 0, "BT": 1, "P": 2}
 DTYPES = {np.dtype(np.int32): 0, np.dtype(np.float32): 1, np.dtype(np.float64): 2}
 

@@ -57,7 +57,11 @@ class GpuSlot:
         return (plan_bytes or 0) + 8 * max_batch * (4 * (2 * N - 1) * N + N * N)
 
     def device_bytes(self):
+        """Device bytes of this slot alone; the dense tables are shared by the slots of one (N, p)."""
         return self.plan.plan_bytes + self.d_b.nbytes + self.d_x.nbytes
+
+    def shared_bytes(self):
+        return getattr(self.plan, "table_bytes", 0)
 
     def launch(self, reqs):
         S = len(reqs)
@@ -108,7 +112,7 @@ class Server:
         for rnd in range(slots_per_key):
             for n, cap in self.keys:
                 s = factory(n, cap, max_batch)
-                b = s.device_bytes()
+                b = s.device_bytes() + (getattr(s, "shared_bytes", lambda: 0)() if rnd == 0 else 0)
                 if mem_budget is not None and used + b > mem_budget and rnd > 0:
                     s.close()  # admission by memory: no further instances of this key
                     continue

@@ -1,17 +1,21 @@
-"""Benchmark: hybrid ADRT inverses per second at N = 256 on one MI355X.
+"""Benchmark: hybrid ADRT inverses per second at N = 256 on one MI355X, plus mixed-N batching and serving.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--n 8] [--dense-cap 16] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--batch B] [--n 8] [--dense-cap 16] [--no-cpu] [--no-serve]
+    python bench.py --rows-only      # short workload for rocprofv3 --pmc runs of the row kernel
 
-A step is one batched hybrid inverse (FP32 row engine, FP64 dense table / preconditioner / outer iteration) of B
-noisy ellipse-phantom sinograms that are resident in HBM, replayed from one hipGraph.  ``value`` is inverses/s
-(B * K / elapsed).  Beside it the line reports the plain FP64 GPU path (one sinogram per call, no graph, no batching),
-the relative error of the FP32 result against the FP64 one, the roofline of the dominant kernel (the fused row kernel,
-timed with HIP events on its own stream) and the reference C++ row engine timed on the host cores.
+A step is one batched hybrid inverse (FP32 row engine with TwoSum-compensated Chebyshev accumulation; FP64 dense
+table GEMM, preconditioner, outer iteration and the last refinement steps) of B noisy ellipse-phantom sinograms that
+are resident in HBM, replayed from one hipGraph.  ``value`` is inverses/s (B * K / elapsed).  Beside it the line
+reports the plain FP64 GPU path (one sinogram per call, no graph, no batching), the relative error of the FP32 result
+against the FP64 one, the roofline of the dominant kernel (the register row kernel of the G_h pass, timed with HIP
+events recorded around it on its own stream), the reference C++ row engine timed on the host cores, a 512-sinogram
+uneven batch in one graph launch and a Poisson serving trace over mixed N with p50 / p99 latency.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,39 +26,51 @@ sys.path.insert(0, REPO)
 
 import importlib  # noqa: E402
 
-PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
+PEAK = {"fp32": 157.3e12, "fp32_nonpacked": 78.6e12, "fp64": 78.6e12, "hbm": 8.0e12}
+PMC_SUMMARY = os.path.join(REPO, "profiles", "rows_reg_kernel_n256_b64_pmc.txt")
 
 
 def pkg(sub):
     return importlib.import_module(f"adrt-pinv_amd.{sub}")
 
 
-def rows_flops(H, S, ops):
-    """Algorithmic FLOPs of one row-kernel launch over S sinograms (R = 4N rows each):
-    8N per butterfly level-row, (d-1)(16 N U + 18 N) + 2N per Chebyshev row, 8 N^2 per dense row."""
-    n, N, Lr = H.n, H.N, H.L
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def rows_kernel_work(H, S, ops):
+    """Algorithmic FLOPs and HBM bytes of one row-kernel launch over S sinograms.  The kernel takes the rows of the
+    non-dense frequencies (both packed pairs of every sinogram); per row: 8N flops per B / BT level and
+    (d - 1)(16 N U + 18 N) + 2N per Chebyshev (P) op over U levels; it reads and writes each row once (complex64)."""
+    N, Lr = H.N, H.L
     dense = np.zeros(Lr, dtype=bool)
     dense[H.eng.dense_k] = True
+    rows_per_k = 2 * S
     lev = sum(hi - lo + 1 for o, lo, hi in ops if o in ("B", "BT"))
-    f = 0.0
+    d = H.deg.astype(float)[~dense]
+    f = rows_per_k * (~dense).sum() * 8.0 * N * lev
     for o, lo, hi in ops:
-        if o != "P":
-            continue
-        U = hi - lo + 1
-        d = H.deg.astype(float)
-        cheb = np.where(dense, 0.0, (np.maximum(d - 1, 0)) * (16 * N * U + 18 * N) + 2 * N)
-        f += cheb.sum() + dense.sum() * 8.0 * N * N
-    f += Lr * 8.0 * N * lev
-    return 2 * S * f  # two packed pairs per frequency
+        if o == "P":
+            U = hi - lo + 1
+            f += rows_per_k * ((np.maximum(d - 1, 0)) * (16 * N * U + 18 * N) + 2 * N).sum()
+    nbytes = 2.0 * rows_per_k * (~dense).sum() * N * 8
+    return f, nbytes
 
 
-def rows_bytes(H, S, elem):
-    return 2.0 * (2 * H.L * S * H.N * elem) + H.eng.n_dense * H.N * H.N * 16.0
+def pmc_traffic():
+    """HBM bytes per launch of the row kernel from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
+    calibrated against a copy of known size), or None."""
+    try:
+        txt = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, flags=re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(H, cap, b, threads, budget_s=20.0):
     """The reference's C++ row engine (cpp/oblique.cpp built into oracle/_ref) under the oracle's NumPy host, same
-    setup scalars, one sinogram; tmin protocol of exp_compiled_tables.py:20-27 within a time budget."""
+    setup scalars, one sinogram; tmin of up to three runs within a time budget (exp_compiled_tables.py:20-27)."""
     from oracle import inverses as oiv
     from oracle import refcpp
 
@@ -67,13 +83,119 @@ def cpu_baseline(H, cap, b, threads, budget_s=20.0):
     R = RefHybrid(H.n, lam_max=H.lam_max, dense_cap=cap, threads=threads, outer=(H.lo, H.hi, H.outer_deg))
     ts = []
     t_all = time.time()
-    while time.time() - t_all < budget_s or len(ts) < 1:
+    x = None
+    while len(ts) < 3 and (not ts or time.time() - t_all + ts[-1] < budget_s):
         t0 = time.time()
         x = R.apply(b)
         ts.append(time.time() - t0)
-        if len(ts) >= 3:
-            break
     return min(ts), len(ts), x
+
+
+def make_sinos(n, count, seed0, noise_seed0, level=0.01):
+    ph = pkg("phantom")
+    R = pkg("radon")
+    N = 1 << n
+    X = np.stack([ph.ellipses(seed0 + i, N) for i in range(count)])
+    B0 = R.adrt(X)
+    return np.stack([ph.add_noise(B0[i], level, noise_seed0 + i) for i in range(count)])
+
+
+def time_rows_kernel(L, H, S, st, reps=10):
+    """Average duration of the row kernel of the G_h pass: events recorded immediately before and after it."""
+    ev0, ev1 = L.Event(), L.Event()
+    L.check(L.load().adrt_plan_time_rows(H._plan, ev0.handle, ev1.handle), "plan_time_rows")
+    try:
+        L.check(L.load().adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")  # warm-up
+        ts = []
+        for _ in range(reps):
+            L.check(L.load().adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")
+            st.sync()
+            ts.append(ev0.elapsed_ms(ev1))
+    finally:
+        L.check(L.load().adrt_plan_time_rows(H._plan, None, None), "plan_time_rows")
+    return float(np.mean(ts)), ts
+
+
+def mixed_batch_leg(count=512, dense_cap=16):
+    """Uneven batch: N drawn from {64, 128, 256, 512} with weights .4/.3/.2/.1 (default_rng(42)), phantoms 1000+i,
+    noise 2000+i at 1%, FP32 hybrid, one graph launch, tuned row mapping per N."""
+    B = pkg("batch")
+    ns = B.draw_sizes(count)
+    ph = pkg("phantom")
+    R = pkg("radon")
+    sinos = [ph.add_noise(R.adrt(ph.ellipses(1000 + i, 1 << n)), 0.01, 2000 + i) for i, n in enumerate(ns)]
+    t0 = time.time()
+    mb = B.MixedBatch(ns, precision="fp32", refine=2, dense_cap=dense_cap)
+    setup_s = time.time() - t0
+    mb.set_inputs(sinos)
+    mb.capture()
+    mb.launch()
+    mb.main.sync()
+    L = pkg("lib")
+    e0, e1 = L.Event(), L.Event()
+    reps = 3
+    e0.record(mb.main)
+    for _ in range(reps):
+        mb.launch()
+    e1.record(mb.main)
+    ms = e0.elapsed_ms(e1) / reps
+    counts = {int(1 << n): ns.count(n) for n in sorted(set(ns))}
+    tuned = {int(1 << n): int(v[0]) for n, v in mb.tuning.items()}
+    mb.close()
+    return {"sinograms": count, "sizes": counts, "ms_per_launch": ms, "inverses_per_s": count * 1000.0 / ms,
+            "setup_s": setup_s, "row_mapping_loge": tuned, "dense_cap": dense_cap}
+
+
+def serving_leg(rate, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), max_batch=4,
+                slots_per_key=2):
+    """Poisson arrivals (default_rng(7)) of FP32 hybrid requests over mixed N and p; p50 / p99 latency."""
+    S = pkg("serve")
+    C = pkg("plancache")
+    keys = [(n, c) for n in sizes for c in caps]
+    t0 = time.time()
+    setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
+    setup_s = time.time() - t0
+    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=max_batch, setups=setups)
+    pool = {n: list(make_sinos(n, 4, 3000, 4000)) for n in sizes}
+    # warm every slot once (first launches load code objects)
+    warm = [(0.0, S.Request(-1 - i, s.n, s.cap, pool[s.n][0], 0.0)) for i, s in enumerate(srv.slots)]
+    srv.run_trace(warm)
+    srv.done.clear()
+    trace = S.poisson_trace(rate, duration, list(sizes), list(weights), list(caps), pool)
+    done = srv.run_trace(trace)
+    st = S.latency_stats(done)
+    out = {"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **st,
+           "mix": "N in {128,256,512,1024} w .4/.3/.2/.1, p in {16,32}", "slots": len(srv.slots),
+           "max_batch": max_batch, "device_GiB": srv.device_bytes / 2**30, "setup_s": setup_s}
+    srv.close()
+    return out
+
+
+def rows_only(a):
+    """Short workload for hardware-counter runs: K launches of the row kernel of the G_h pass at the headline
+    configuration, then one device-to-device copy of a known size (the counter calibration)."""
+    L = pkg("lib")
+    I = pkg("inverses")
+    n, N, B = a.n, 1 << a.n, a.batch
+    cap = a.dense_cap if a.dense_cap >= 0 else None
+    H64 = I.FastHybridHip(n, dense_cap=cap, precision="fp64", max_batch=1)
+    H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B,
+                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=a.refine)
+    st = L.Stream()
+    d_x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal((B, N, N)), st)
+    d_y = L.DeviceArray((B, N, N))
+    H32.operator_device(d_x, d_y, B, st)
+    for _ in range(5):
+        L.check(L.load().adrt_plan_rows_pass(H32._plan, 1, B, st.handle), "rows_pass")
+    copy_bytes = 1 << 28
+    src = L.DeviceArray((copy_bytes // 8,))
+    dst = L.DeviceArray((copy_bytes // 8,))
+    src.zero(st)
+    L.check(L.load().adrt_memcpy(dst.ptr, src.ptr, copy_bytes, 3, st.handle), "d2d")
+    st.sync()
+    fl, nb = rows_kernel_work(H32, B, H32.ops_op())
+    print(json.dumps({"rows_kernel_launches": 5, "flops_per_launch": fl, "alg_bytes_per_launch": nb,
+                      "calibration_copy_bytes": copy_bytes}), flush=True)
 
 
 def main():
@@ -88,6 +210,11 @@ def main():
     ap.add_argument("--plain-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-duration", type=float, default=2.0)
+    ap.add_argument("--rows-only", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the sinogram batch would shard by "
@@ -96,26 +223,24 @@ def main():
 
     L = pkg("lib")
     I = pkg("inverses")
-    ph = pkg("phantom")
-    R = pkg("radon")
-    L.load()
     if L.load().adrt_device_count() < 1:
         raise SystemExit("no GPU visible")
+    if a.rows_only:
+        rows_only(a)
+        return
     n, N, B = a.n, 1 << a.n, a.batch
     cap = a.dense_cap if a.dense_cap >= 0 else None
 
     t0 = time.time()
     H64 = I.FastHybridHip(n, dense_cap=cap, precision="fp64", max_batch=1)
     setup_s = time.time() - t0
-    H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B,
-                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=a.refine)
-    print(f"# setup {setup_s:.1f}s: D={H64.outer_deg} sum(deg)/N={H64.deg.sum() / N:.1f} dense rows={H64.eng.n_dense}",
-          file=sys.stderr, flush=True)
+    outer = (H64.lo, H64.hi, H64.outer_deg)
+    H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
+                          refine=a.refine)
+    log(f"setup {setup_s:.1f}s: D={H64.outer_deg} sum(deg)/N={H64.deg.sum() / N:.1f} dense rows={H64.eng.n_dense}")
 
     # inputs: ellipse phantoms (seeds 1000+i) through the GPU forward ADRT, plus 1% noise (seeds 2000+i)
-    X = np.stack([ph.ellipses(1000 + i, N) for i in range(B)])
-    B0 = R.adrt(X)
-    Bn = np.stack([ph.add_noise(B0[i], 0.01, 2000 + i) for i in range(B)])
+    Bn = make_sinos(n, B, 1000, 2000)
     st = L.Stream()
     d_b = L.DeviceArray.from_host(Bn, st)
     d_x32 = L.DeviceArray((B, N, N))
@@ -134,39 +259,58 @@ def main():
     e1.record(st)
     plain_ms = e0.elapsed_ms(e1) / a.plain_reps
     plain_ips = 1000.0 / plain_ms
+    log(f"plain fp64 {plain_ms:.2f} ms per inverse")
 
-    # batched FP32 path in one hipGraph
-    with L.Graph(st) as g:
-        H32.apply_device(d_b, d_x32, B, st, count=False)
-    for _ in range(a.warmup):
-        g.launch(st)
-    st.sync()
-    x32 = d_x32.download(st)
-    rel = np.linalg.norm((x32 - x64).reshape(B, -1), axis=1) / np.linalg.norm(x64.reshape(B, -1), axis=1)
-    t_wall = time.time()
-    e0.record(st)
-    for _ in range(a.steps):
-        g.launch(st)
-    e1.record(st)
-    step_ms = e0.elapsed_ms(e1) / a.steps
-    wall_ms = (time.time() - t_wall) * 1000.0 / a.steps
+    def batched(H):
+        with L.Graph(st) as g:
+            H.apply_device(d_b, d_x32, B, st, count=False)
+        for _ in range(a.warmup):
+            g.launch(st)
+        st.sync()
+        x = d_x32.download(st)
+        rel = np.linalg.norm((x - x64).reshape(B, -1), axis=1) / np.linalg.norm(x64.reshape(B, -1), axis=1)
+        t_wall = time.time()
+        e0.record(st)
+        for _ in range(a.steps):
+            g.launch(st)
+        e1.record(st)
+        ms = e0.elapsed_ms(e1) / a.steps
+        return ms, (time.time() - t_wall) * 1000.0 / a.steps, rel
+
+    step_ms, wall_ms, rel = batched(H32)
     ips = B * 1000.0 / step_ms
+    log(f"batched fp32 {step_ms:.2f} ms per step of {B}: {ips:.0f} inverses/s, relerr max {rel.max():.2e}")
 
-    # dominant kernel: the fused row kernel of the G_h pass, on the same stream, events around K launches
+    # ablation: the same batch without the TwoSum compensation
+    H32p = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32_plain", max_batch=B, outer=outer,
+                           refine=a.refine)
+    plain32_ms, _, rel_p = batched(H32p)
+    H32p.close()
+
+    # dominant kernel: the register row kernel of the G_h pass, events around the kernel on its own stream
     ops = H32.ops_op()
     d_tmp = L.DeviceArray((B, N, N))
-    H32.operator_device(d_x32, d_tmp, B, st)  # fills the plan's rows with a real G_h pass
-    K = 10
-    e0.record(st)
-    for _ in range(K):
-        L.check(L.load().adrt_plan_rows_pass(H32._plan, 1, B, st.handle), "rows_pass")
-    e1.record(st)
-    rows_ms = e0.elapsed_ms(e1) / K
-    fl = rows_flops(H32, B, ops)
+    H32.operator_device(d_x32, d_tmp, B, st)  # leaves the rows of a real G_h pass in the plan
+    rows_ms, _ = time_rows_kernel(L, H32, B, st)
+    fl, nb = rows_kernel_work(H32, B, ops)
     achieved = fl / (rows_ms * 1e-3)
     roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK["fp32"], "traffic": None, "kernel": "rows_kernel<float,float>",
-            "kernel_ms": rows_ms, "flops_per_launch": fl, "bytes_per_launch": rows_bytes(H32, B, 8)}
+            "frac": achieved / PEAK["fp32"], "traffic": pmc_traffic(),
+            "frac_nonpacked": achieved / PEAK["fp32_nonpacked"],
+            "kernel": f"rows_reg_kernel<float, {n}, e, comp> (G_h pass)", "kernel_ms": rows_ms,
+            "flops_per_launch": fl, "alg_bytes_per_launch": nb, "units_per_launch": f"{2 * B} rows x "
+            f"{H32.L - H32.eng.n_dense} non-dense frequencies"}
+    log(f"row kernel {rows_ms * 1e3:.1f} us: {achieved / 1e12:.1f} TF/s")
+
+    mixed = None
+    if not a.no_mixed:
+        mixed = mixed_batch_leg()
+        log(f"mixed batch: {mixed['inverses_per_s']:.0f} inverses/s")
+    serving = None
+    if not a.no_serve:
+        serving = serving_leg(a.serve_rate, a.serve_duration)
+        log(f"serving at {a.serve_rate:.0f}/s: p50 {serving.get('p50_ms', 0):.1f} ms p99 "
+            f"{serving.get('p99_ms', 0):.1f} ms")
 
     cpu = None
     if not a.no_cpu:
@@ -175,9 +319,11 @@ def main():
         if res is not None:
             tmin, reps, xr = res
             cpu = {"value": 1.0 / tmin, "unit": "inverses/s", "cores": threads, "kind": "reference",
-                   "sample": f"1 hybrid inverse at N={N} (dense_cap={cap}), tmin of {reps} after none; reference "
-                             f"cpp/oblique.cpp kernels (OpenMP, {threads} threads) under the oracle's NumPy host",
+                   "sample": f"1 hybrid inverse at N={N} (dense_cap={cap}, D={H64.outer_deg}), tmin of {reps}; "
+                             f"reference cpp/oblique.cpp row kernels (OpenMP, {threads} threads) under the oracle's "
+                             "NumPy host (FFTs, packing, outer loop)",
                    "relerr_vs_gpu_fp64": float(np.linalg.norm(xr - x64[0]) / np.linalg.norm(x64[0]))}
+            log(f"cpu baseline {tmin:.2f} s per inverse")
 
     line = {
         "metric": "hybrid inverses/s at N=256, FP32 vs FP64 (relerr<=1e-5); p99 latency mixed-N",
@@ -190,7 +336,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 rows + fp64 dense/precond/outer",
+        "dtype": "f32 rows (TwoSum-compensated) + f64 dense GEMM/precond/outer/refine",
         "data": "synthetic: ellipse phantoms (seeds 1000+i) + 1% Gaussian noise (seeds 2000+i)",
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
                    "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
@@ -199,9 +345,13 @@ def main():
         "speedup_vs_plain_fp64": ips / plain_ips,
         "relerr_fp32_vs_fp64_max": float(rel.max()),
         "relerr_fp32_vs_fp64_median": float(np.median(rel)),
+        "fp32_plain": {"inverses_per_s": B * 1000.0 / plain32_ms, "relerr_max": float(rel_p.max()),
+                       "relerr_median": float(np.median(rel_p))},
         "wall_ms_per_step": wall_ms,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed_batch": mixed,
+        "serving": serving,
     }
     if cpu:
         line["speedup_vs_cpu"] = ips / cpu["value"]

@@ -73,7 +73,8 @@ int adrt_event_query(void* ev); /* 1 done, 0 pending, < 0 error */
 
 /* Row engine with device-resident parameters: the plug-in surface of fastops.RowEngine (fastops.py:20-167).
  * Row r of a run uses parameter row r % L.  Rows whose parameter row is listed in dense_k use the exact filter
- * dense_F[i] (N x N, row-major, complex128) in P ops.  precision 0: complex128 rows; 1: complex64 rows.
+ * dense_F[i] (N x N, row-major, complex128) in P ops.  precision 0: complex128 rows; 1: complex64 rows with the
+ * TwoSum-compensated Chebyshev iterate; 2: complex64 rows without the compensation.
  * ops: nops triples (kind, lo, hi), kind 0 = B, 1 = BT, 2 = P, applied left to right (fastops.py:156-167). */
 typedef struct adrt_rows_plan adrt_rows_plan;
 int adrt_rows_plan_create(adrt_rows_plan** out, int n, int L, const adrt_c128* z, const int* deg, const double* a,
@@ -93,12 +94,14 @@ int adrt_rows_config_get(int n, int precision);
  *   b: (S, 4, 2N-1, N) sinograms, x / y: (S, N, N) images. */
 enum { ADRT_KIND_LP = 0, ADRT_KIND_HYBRID = 1 };
 enum { ADRT_PRECOND_NONE = 0, ADRT_PRECOND_G1 = 1, ADRT_PRECOND_LORR = 2 };
-enum { ADRT_PREC_F64 = 0, ADRT_PREC_F32 = 1 };
+enum { ADRT_PREC_F64 = 0, ADRT_PREC_F32 = 1, ADRT_PREC_F32_PLAIN = 2 };
 typedef struct {
     int kind;           /* ADRT_KIND_* */
     int n;              /* N = 2^n, 1 <= n <= 11 (hybrid: n >= 2) */
     int precision;      /* ADRT_PREC_F64: every op in double.  ADRT_PREC_F32: offset FFTs, butterflies and Chebyshev
-                           steps in float; dense table, preconditioner and outer iteration in double */
+                           steps in float, the Chebyshev iterate accumulated with TwoSum compensation; dense table,
+                           preconditioner and outer iteration in double.  ADRT_PREC_F32_PLAIN: ADRT_PREC_F32 without
+                           the compensation */
     int precond;        /* hybrid only: ADRT_PRECOND_* */
     int g1_deg;         /* Chebyshev degree of G_1^{-1} (fastops default 14) */
     int outer_deg;      /* D, the outer Chebyshev degree */
@@ -120,7 +123,9 @@ typedef struct {
     const adrt_c128* dense_Wback;
     int refine;         /* ADRT_PREC_F32 only: the backprojection and the last `refine` outer steps run in FP64 */
     int dense_on_device; /* 1: dense_F / dense_Wop / dense_Wback are DEVICE pointers to column-major matrices
-                            (the row-major transposes), copied without a host round trip */
+                            (the row-major transposes), copied without a host round trip.  2: the same, but the plan
+                            borrows them (no copy): the caller keeps them alive until adrt_plan_destroy, and plans of
+                            one size share one set of tables */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;
@@ -138,6 +143,9 @@ int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, vo
 /* the row kernel alone on the rows of the plan's last packed pass (which: 0 backproject ops, 1 operator ops), written
  * to a second buffer: the dominant kernel of an apply, for timing */
 int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream);
+/* every later launch of the plan's row kernel is bracketed by records of the two events (NULL, NULL: stop), so the
+ * kernel alone can be timed inside a pass; not for a plan whose work is being captured into a graph */
+int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end);
 /* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
 
@@ -155,6 +163,17 @@ unsigned long long adrt_work_bytes_fwd(int n, int dtype, int S);
 unsigned long long adrt_work_bytes_T(int n, int dtype, int S);
 int adrt_forward(int n, int dtype, const void* d_x, void* d_b, void* d_work, int S, void* stream);
 int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, int S, void* stream);
+
+/* Least squares min ||R x - b||_2 by CGLS on the GPU: pinv.cgls (pinv.py:23-48), the cross-check of the inverses,
+ * batched over S sinograms that each stop on their own test ||R^T r_j|| <= tol ||R^T r_0||.
+ * d_b: (S, 4, 2N-1, N) and d_x: (S, N, N) float64 on the device; x0 = 1 starts from d_x, 0 from zero.  iters (host,
+ * S ints) receives the iterations each sinogram ran; hist (host, S x hist_len doubles, NULL when hist_len = 0) the
+ * history ||R^T r_j||, j = 0..iters (the reference's return value).  Where R^T b = 0 the result is x = 0 after no
+ * iteration (the reference divides 0 by 0 there).  d_work: adrt_cgls_work_bytes(n, S, hist_len) bytes.  Synchronises
+ * the stream once per iteration, so it cannot be captured into a graph. */
+unsigned long long adrt_cgls_work_bytes(int n, int S, int hist_len);
+int adrt_cgls(int n, const double* d_b, double* d_x, int S, double tol, int maxit, int x0, void* d_work, int* iters,
+              double* hist, int hist_len, void* stream);
 
 #ifdef __cplusplus
 }

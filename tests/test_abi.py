@@ -49,3 +49,14 @@ def test_plan_desc_layout():
     (6 ints, 3 doubles, 5 pointers, int + pad, 2 pointers, int + pad, 2 pointers, 2 ints)."""
     L = pkg("lib")
     assert ctypes.sizeof(L.PlanDesc) == 6 * 4 + 3 * 8 + 5 * 8 + 8 + 2 * 8 + 8 + 2 * 8 + 8
+
+
+def test_every_package_module_imports():
+    """Every host module of the package imports without a GPU (syntax, imports, ctypes tables)."""
+    import glob
+
+    mods = sorted(os.path.basename(p)[:-3] for p in glob.glob(os.path.join(REPO, "adrt-pinv_amd", "*.py")))
+    for m in mods:
+        if m in ("__init__", "build"):
+            continue
+        pkg(m)
