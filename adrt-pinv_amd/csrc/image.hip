@@ -67,12 +67,18 @@ __global__ void lorr_kernel(int n, const double* y, double* out, long long count
     }
 }
 
-// g1_inv fused into one launch: each block owns a TILE x TILE output tile and iterates the Chebyshev solve of G_1 on
-// the tile grown by a halo of (deg - 1) pixels, so every output pixel sees exact neighbours through all steps (the
-// stencil reaches one pixel per step).  The current iterate lives in LDS (neighbour reads); y, the previous iterate
-// and the new values stay in registers.  Pixels outside the image are held at zero like the stencil's zero extension.
-// The epilogue optionally scales (mode 1) or applies the outer Chebyshev update (mode 2) so P and the outer step cost
-// one pass over the image.
+// g1_inv fused into one launch.  A block owns a TILE x TILE output tile and iterates the Chebyshev solve of G_1 on the
+// tile grown by a halo of HALO >= deg - 1 pixels (even, so the region starts on an even pixel and 2 x 2 cells are
+// aligned with G_1's pair structure): the stencil reaches one pixel per step, so after step k only the tile dilated
+// by (deg - 1 - k) must be exact, and the cells outside that shrinking square are skipped.
+//
+// Each thread owns 2 x 2 cells (up to BPT of them) and keeps their y, current and previous iterate in registers.  For
+// a cell (i, j), G_1 x = 8 x + [1 2 1] down the partner column j^1 (rows i-1..i+1) + [1 2 1] along the partner row i^1
+// (columns j-1..j+1) (fasthybrid.py:81-98); inside a 2 x 2 cell both partners are in the cell, so a step reads only
+// eight neighbours from LDS (rows above / below, columns left / right of the cell) instead of seven per pixel.  The
+// iterate is double-buffered in LDS (step k reads buffer (k-1)&1 and writes k&1), one barrier per step.  Pixels
+// outside the image and a one-pixel border around the region hold zero, the stencil's zero extension.  The epilogue
+// optionally scales (mode 1) or applies the outer Chebyshev update (mode 2), so P and the outer step cost one pass.
 constexpr int G1_MAXH = 16;
 constexpr int G1_THREADS = 512;
 
@@ -86,87 +92,119 @@ struct G1Args {
     double om[32];     // inner omega_k, k = 1..deg-1
 };
 
-// Region slot e of a thread is (ri, rj) = divmod(tid + 512 e, REG); the region of a launch is the top-left R x R of
-// the REG x REG slot square (R = TILE + 2 H, REG = TILE + 32).  TILE 64 halves the halo overhead; TILE 32 gives a
-// small batch more blocks.
-template <int G1_TILE>
+template <int TILE, int HALO>
+struct G1Cfg {
+    static constexpr int R = TILE + 2 * HALO;  // region side (even)
+    static constexpr int RB = R / 2;           // 2 x 2 cells per region side
+    static constexpr int RP = R + 2;           // LDS pitch: region plus a zero border on each side
+    static constexpr int BUF = RP * RP;
+    static constexpr int NB = RB * RB;
+    static constexpr int BPT = (NB + G1_THREADS - 1) / G1_THREADS;
+    static constexpr size_t LDS = 2 * BUF * sizeof(double);
+};
+
+template <int TILE, int HALO>
 __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
-    constexpr int G1_REG = G1_TILE + 2 * G1_MAXH;
-    constexpr int G1_RP = G1_REG + 2;  // LDS row pitch: region plus a zero border
-    constexpr int G1_PER = G1_REG * G1_REG / G1_THREADS;
-    static_assert(G1_REG * G1_REG % G1_THREADS == 0, "region slots must tile the block");
-    extern __shared__ double X[];  // G1_RP * G1_RP
+    using C = G1Cfg<TILE, HALO>;
+    constexpr int RB = C::RB, RP = C::RP, BUF = C::BUF, NB = C::NB, BPT = C::BPT;
+    extern __shared__ double X[];  // two buffers of BUF doubles
     const int N = 1 << A.n;
-    const int H = A.deg - 1;
-    const int R = G1_TILE + 2 * H;
+    const int H = A.deg - 1;  // <= HALO
     const int s = blockIdx.z;
-    const int ti0 = blockIdx.y * G1_TILE - H, tj0 = blockIdx.x * G1_TILE - H;
+    const int gi0 = blockIdx.y * TILE - HALO, gj0 = blockIdx.x * TILE - HALO;
     const double* ys = A.y + (size_t)s * N * N;
     const double a = 2.0 / 20.0;
-    for (int i = threadIdx.x; i < G1_RP * G1_RP; i += G1_THREADS) X[i] = 0.0;
+    for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = 0.0;
     __syncthreads();
-    double yv[G1_PER], xp[G1_PER];  // the current iterate lives in LDS only
-    unsigned inmask = 0;
+    double yv[BPT][4], xc[BPT][4], xq[BPT][4];
+    unsigned inm[BPT];
 #pragma unroll
-    for (int e = 0; e < G1_PER; ++e) {
-        const int idx = threadIdx.x + G1_THREADS * e;
-        const int ri = idx / G1_REG, rj = idx % G1_REG;
-        const int gi = ti0 + ri, gj = tj0 + rj;
-        const bool in = ri < R && rj < R && gi >= 0 && gi < N && gj >= 0 && gj < N;
-        inmask |= (in ? 1u : 0u) << e;
-        yv[e] = in ? ys[(size_t)gi * N + gj] : 0.0;
-        xp[e] = 0.0;
-        X[(ri + 1) * G1_RP + rj + 1] = a * yv[e];
+    for (int e = 0; e < BPT; ++e) {
+        const int b = threadIdx.x + G1_THREADS * e;
+        const int bi = b / RB, bj = b % RB;
+        inm[e] = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ri = 2 * bi + (c >> 1), rj = 2 * bj + (c & 1);
+            const int gi = gi0 + ri, gj = gj0 + rj;
+            const bool in = b < NB && gi >= 0 && gi < N && gj >= 0 && gj < N;
+            inm[e] |= (in ? 1u : 0u) << c;
+            yv[e][c] = in ? ys[(size_t)gi * N + gj] : 0.0;
+            xc[e][c] = a * yv[e][c];
+            xq[e][c] = 0.0;
+            if (b < NB) X[(ri + 1) * RP + rj + 1] = xc[e][c];
+        }
     }
     __syncthreads();
-    for (int k = 1; k < A.deg; ++k) {
-        const double om = A.om[k - 1];
-        double xn[G1_PER];
+    for (int k = 1; k <= H; ++k) {
+        const double om = A.om[k - 1], om1 = om - 1.0;
+        const double* Xr = X + ((k - 1) & 1) * BUF;
+        double* Xw = X + (k & 1) * BUF;
+        // region rows / columns that must be exact after step k
+        const int lo = HALO - (H - k), hi = HALO + TILE + (H - k);
 #pragma unroll
-        for (int e = 0; e < G1_PER; ++e) {
-            const int idx = threadIdx.x + G1_THREADS * e;
-            const int ri = idx / G1_REG, rj = idx % G1_REG;
-            // parity of the global index equals the parity of the region index shifted by the (even) origin offset
-            const int pi = (ti0 + ri) & 1, pj = (tj0 + rj) & 1;
-            // pixels outside the image hold 0 in X and the border row/column is 0, so every neighbour read is plain
-            const int c = (ri + 1) * G1_RP + rj + 1;
-            const int cj = c + (pj ? -1 : 1), ci = c + (pi ? -G1_RP : G1_RP);
-            const double xk = X[c];
-            const double t1 = 2.0 * X[cj] + X[cj - G1_RP] + X[cj + G1_RP];
-            const double t2 = 2.0 * X[ci] + X[ci - 1] + X[ci + 1];
-            const double g = 8.0 * xk + t1 + t2;
-            const double r = yv[e] - g;
-            xn[e] = ((inmask >> e) & 1) ? xk + om * (a * r) + (om - 1.0) * (xk - xp[e]) : 0.0;
-            xp[e] = xk;
+        for (int e = 0; e < BPT; ++e) {
+            const int b = threadIdx.x + G1_THREADS * e;
+            const int r0 = 2 * (b / RB), c0 = 2 * (b % RB);
+            if (b >= NB || r0 + 1 < lo || r0 >= hi || c0 + 1 < lo || c0 >= hi) continue;
+            const int o = (r0 + 1) * RP + c0 + 1;  // LDS index of cell corner (r0, c0)
+            const double up0 = Xr[o - RP], up1 = Xr[o - RP + 1];
+            const double dn0 = Xr[o + 2 * RP], dn1 = Xr[o + 2 * RP + 1];
+            const double lf0 = Xr[o - 1], lf1 = Xr[o + RP - 1];
+            const double rt0 = Xr[o + 2], rt1 = Xr[o + RP + 2];
+            const double x00 = xc[e][0], x01 = xc[e][1], x10 = xc[e][2], x11 = xc[e][3];
+            double g[4];
+            g[0] = 8.0 * x00 + (up1 + 2.0 * x01 + x11) + (lf1 + 2.0 * x10 + x11);
+            g[1] = 8.0 * x01 + (up0 + 2.0 * x00 + x10) + (x10 + 2.0 * x11 + rt1);
+            g[2] = 8.0 * x10 + (x01 + 2.0 * x11 + dn1) + (lf0 + 2.0 * x00 + x01);
+            g[3] = 8.0 * x11 + (x00 + 2.0 * x10 + dn0) + (x00 + 2.0 * x01 + rt0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double xo = xc[e][c];
+                const double xn = ((inm[e] >> c) & 1) ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : 0.0;
+                xq[e][c] = xo;
+                xc[e][c] = xn;
+            }
+            Xw[o] = xc[e][0];
+            Xw[o + 1] = xc[e][1];
+            Xw[o + RP] = xc[e][2];
+            Xw[o + RP + 1] = xc[e][3];
         }
         __syncthreads();
-#pragma unroll
-        for (int e = 0; e < G1_PER; ++e) {
-            const int idx = threadIdx.x + G1_THREADS * e;
-            const int ri = idx / G1_REG, rj = idx % G1_REG;
-            X[(ri + 1) * G1_RP + rj + 1] = xn[e];
-        }
-        __syncthreads();
     }
-    // epilogue on the tile's interior
+    // epilogue on the tile's interior, from the registers
 #pragma unroll
-    for (int e = 0; e < G1_PER; ++e) {
-        const int idx = threadIdx.x + G1_THREADS * e;
-        const int ri = idx / G1_REG, rj = idx % G1_REG;
-        const int gi = ti0 + ri, gj = tj0 + rj;
-        if (ri < H || rj < H || ri >= H + G1_TILE || rj >= H + G1_TILE || gi >= N || gj >= N) continue;
-        const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
-        const double pv = X[(ri + 1) * G1_RP + rj + 1];
-        if (A.mode == 0) {
-            A.out[o] = pv;
-        } else if (A.mode == 1) {
-            A.out[o] = A.a_out * pv;
-        } else {
-            const double xo = A.x[o];
-            const double dx = A.xp ? (xo - A.xp[o]) : (xo - 0.0);
-            A.out[o] = xo + A.w1 * (A.a_out * pv) + A.w0 * dx;
+    for (int e = 0; e < BPT; ++e) {
+        const int b = threadIdx.x + G1_THREADS * e;
+        if (b >= NB) continue;
+        const int bi = b / RB, bj = b % RB;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ri = 2 * bi + (c >> 1), rj = 2 * bj + (c & 1);
+            const int gi = gi0 + ri, gj = gj0 + rj;
+            if (ri < HALO || rj < HALO || ri >= HALO + TILE || rj >= HALO + TILE || gi >= N || gj >= N) continue;
+            const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
+            const double pv = xc[e][c];
+            if (A.mode == 0) {
+                A.out[o] = pv;
+            } else if (A.mode == 1) {
+                A.out[o] = A.a_out * pv;
+            } else {
+                const double xo = A.x[o];
+                const double dx = A.xp ? (xo - A.xp[o]) : (xo - 0.0);
+                A.out[o] = xo + A.w1 * (A.a_out * pv) + A.w0 * dx;
+            }
         }
     }
+}
+
+template <int TILE, int HALO>
+int launch_g1_cfg(const G1Args& A, int S, hipStream_t st) {
+    const int N = 1 << A.n;
+    const int T = (N + TILE - 1) / TILE;
+    constexpr size_t lds = G1Cfg<TILE, HALO>::LDS;
+    hipLaunchKernelGGL((g1_fused_kernel<TILE, HALO>), dim3(T, T, S), dim3(G1_THREADS), lds, st, A);
+    return launch_check("g1_fused_kernel");
 }
 
 }  // namespace
@@ -194,17 +232,10 @@ int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, 
         A.om[k - 1] = om;
     }
     const int N = 1 << n;
-    const long long tiles64 = (long long)S * ((N + 63) / 64) * ((N + 63) / 64);
-    if (tiles64 >= 256) {
-        const int T = (N + 63) / 64;
-        constexpr int RP = 64 + 2 * G1_MAXH + 2;
-        hipLaunchKernelGGL(g1_fused_kernel<64>, dim3(T, T, S), dim3(G1_THREADS), sizeof(double) * RP * RP, st, A);
-    } else {
-        const int T = (N + 31) / 32;
-        constexpr int RP = 32 + 2 * G1_MAXH + 2;
-        hipLaunchKernelGGL(g1_fused_kernel<32>, dim3(T, T, S), dim3(G1_THREADS), sizeof(double) * RP * RP, st, A);
-    }
-    return launch_check("g1_fused_kernel");
+    // TILE 64 halves the halo overhead; TILE 32 gives a small batch more blocks
+    const bool big = (long long)S * ((N + 63) / 64) * ((N + 63) / 64) >= 256;
+    if (deg - 1 <= 14) return big ? launch_g1_cfg<64, 14>(A, S, st) : launch_g1_cfg<32, 14>(A, S, st);
+    return big ? launch_g1_cfg<64, 16>(A, S, st) : launch_g1_cfg<32, 16>(A, S, st);
 }
 
 int launch_scale(const double* x, double alpha, double* out, long long count, hipStream_t st) {
