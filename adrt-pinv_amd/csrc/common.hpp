@@ -33,6 +33,15 @@ template <typename T>
 __host__ __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
 template <typename T, typename U>
 __host__ __device__ inline cplx<T> cast(cplx<U> a) { return {(T)a.x, (T)a.y}; }
+// c + a * b and c + conj(a) * b as two FMAs per component
+template <typename T>
+__device__ inline cplx<T> cfma(cplx<T> a, cplx<T> b, cplx<T> c) {
+    return {fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))};
+}
+template <typename T>
+__device__ inline cplx<T> cfmac(cplx<T> a, cplx<T> b, cplx<T> c) {
+    return {fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y))};
+}
 
 // Error-free sum (Knuth's TwoSum): s = fl(a + b) and e with a + b = s + e exactly in round-to-nearest.  Additions
 // only, so contraction into FMAs cannot alter it.

@@ -71,7 +71,9 @@ struct Row {
     cplx<T> z, zc;
 };
 
-// ---- one pass of levels, forward / adjoint, in registers
+// ---- one pass of levels, forward / adjoint, in registers.  In level i of a pass, the pairs of a lane that share a
+// twiddle t (same group gs, same low bits jl) also share t z, computed once: the forward pair is then
+// (f0 + t f1, f0 + (t z) f1) and the adjoint (g0 + g1, conj(t) g0 + conj(t z) g1), two FMAs per real component.
 template <typename T, int LOGN, int LOGE, int K>
 __device__ __forceinline__ void pass_fwd(cplx<T> (&v)[1 << LOGE], const cplx<T>* tw, cplx<T> z, int lo, int hi) {
     using C = Cfg<T, LOGN, LOGE>;
@@ -83,19 +85,21 @@ __device__ __forceinline__ void pass_fwd(cplx<T> (&v)[1 << LOGE], const cplx<T>*
         const int m = B0 + i + 1;
         if (m < lo || m > hi) continue;
 #pragma unroll
-        for (int u0 = 0; u0 < C::E; ++u0) {
-            if (u0 & (1 << i)) continue;
-            const int u1 = u0 | (1 << i);
-            const int gs = u0 >> KB;
-            const int jl = u0 & ((1 << i) - 1);
-            const cplx<T> f0 = v[u0], f1 = v[u1];
-            cplx<T> u;
-            if (B0 == 0 && jl == 0)
-                u = f1;  // t = 0
-            else
-                u = mul(tw[gs * PER + (1 << i) - 1 + jl], f1);
-            v[u0] = add(f0, u);
-            v[u1] = add(f0, mul(u, z));
+        for (int gs = 0; gs < (C::E >> KB); ++gs) {
+#pragma unroll
+            for (int jl = 0; jl < (1 << i); ++jl) {
+                const bool one = B0 == 0 && jl == 0;  // t = z^0
+                const cplx<T> t = one ? mk((T)1, (T)0) : tw[gs * PER + (1 << i) - 1 + jl];
+                const cplx<T> tz = one ? z : mul(t, z);
+#pragma unroll
+                for (int h = 0; h < (1 << (KB - 1 - i)); ++h) {
+                    const int u0 = (gs << KB) | (h << (i + 1)) | jl;
+                    const int u1 = u0 | (1 << i);
+                    const cplx<T> f0 = v[u0], f1 = v[u1];
+                    v[u0] = one ? add(f0, f1) : cfma(t, f1, f0);
+                    v[u1] = cfma(tz, f1, f0);
+                }
+            }
         }
     }
 }
@@ -106,23 +110,27 @@ __device__ __forceinline__ void pass_adj(cplx<T> (&v)[1 << LOGE], const cplx<T>*
     constexpr int KB = C::kb(K);
     constexpr int B0 = K * LOGE;
     constexpr int PER = (1 << KB) - 1;
+    const cplx<T> z = conj(zc);
 #pragma unroll
     for (int i = KB - 1; i >= 0; --i) {
         const int m = B0 + i + 1;
         if (m < lo || m > hi) continue;
 #pragma unroll
-        for (int u0 = 0; u0 < C::E; ++u0) {
-            if (u0 & (1 << i)) continue;
-            const int u1 = u0 | (1 << i);
-            const int gs = u0 >> KB;
-            const int jl = u0 & ((1 << i) - 1);
-            const cplx<T> g0 = v[u0], g1 = v[u1];
-            v[u0] = add(g0, g1);
-            const cplx<T> s = add(g0, mul(zc, g1));
-            if (B0 == 0 && jl == 0)
-                v[u1] = s;
-            else
-                v[u1] = cmulc(tw[gs * PER + (1 << i) - 1 + jl], s);
+        for (int gs = 0; gs < (C::E >> KB); ++gs) {
+#pragma unroll
+            for (int jl = 0; jl < (1 << i); ++jl) {
+                const bool one = B0 == 0 && jl == 0;
+                const cplx<T> t = one ? mk((T)1, (T)0) : tw[gs * PER + (1 << i) - 1 + jl];
+                const cplx<T> tz = one ? z : mul(t, z);
+#pragma unroll
+                for (int h = 0; h < (1 << (KB - 1 - i)); ++h) {
+                    const int u0 = (gs << KB) | (h << (i + 1)) | jl;
+                    const int u1 = u0 | (1 << i);
+                    const cplx<T> g0 = v[u0], g1 = v[u1];
+                    v[u0] = add(g0, g1);
+                    v[u1] = one ? cfmac(tz, g1, g0) : cfmac(tz, g1, cmulc(t, g0));
+                }
+            }
         }
     }
 }
