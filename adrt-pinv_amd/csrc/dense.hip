@@ -101,81 +101,77 @@ int launch_n(int n, int L, int SP, int n_dense, const int* dense_k, const c128* 
 
 // ---------------------------------------------------------------- MFMA version (FP64 matrix cores)
 // C[r][i] = sum_j Y[r][j] WT[j][i] for the SP rows r of one dense frequency, complex, as four real
-// v_mfma_f64_16x16x4 products per k-step (C_re += Yr Wr - Yi Wi, C_im += Yr Wi + Yi Wr).  A block computes a 64 x 64
-// output tile with four waves of 32 x 32 (2 x 2 MFMA tiles); k runs in chunks of 16 staged in LDS.  Operand layouts of
-// the 16x16x4 f64 instruction (checked on gfx950): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
+// v_mfma_f64_16x16x4 products per k-step (C_re += Yr Wr - Yi Wi, C_im += Yr Wi + Yi Wr).  Operand layouts of the
+// 16x16x4 f64 instruction (checked on gfx950): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
 // D[4 r + lane / 16][lane % 16] in accumulator element r.
+//
+// Each wave owns one 16 x 16 output tile (16 rows of one frequency, 16 outputs) and streams its operands from
+// L2 / HBM straight into the MFMA layout, MF_U k-steps per group with the next group's loads in flight while the
+// current group's 4 MF_U MFMAs run.  A block is four waves on 64 consecutive rows and the same 16 outputs, so the
+// block's W^T columns are fetched once into L1 and shared.  At N = 256 with 128 rows and 33 dense frequencies that is
+// 4224 waves of equal work, about four per SIMD, so the tail costs one wave's time.  Blocks are numbered so that the
+// blocks of one frequency run on one XCD (the dispatcher deals consecutive blocks to the 8 XCDs in turn): the
+// frequency's W^T then comes from HBM once per XCD and from that XCD's L2 for its other row groups.  The output must
+// not alias the input: other blocks still read the rows of the frequency.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int MF_BM = 64, MF_BN = 64, MF_KC = 16;
+constexpr int MF_U = 4;      // k-steps (of 4) per load group
+constexpr int MF_XCD = 8;    // XCDs per device
 
 template <typename Tg>
 __global__ __launch_bounds__(256) void dense_mfma_kernel(int n, int L, int SP, const int* dense_k, const c128* WT,
-                                                         const void* Yin, void* Zout) {
-    __shared__ c128 As[MF_BM][MF_KC + 1];
-    __shared__ c128 Bs[MF_KC][MF_BN + 1];
+                                                         const void* Yin, void* Zout, int gx, int gy, int nblk) {
     const int N = 1 << n;
-    const int slot = blockIdx.z;
+    // logical block id: the hardware id's XCD (id % 8) selects a contiguous eighth of the logical ids
+    const int hw = blockIdx.x, per = (nblk + MF_XCD - 1) / MF_XCD;
+    const int lb = (hw % MF_XCD) * per + hw / MF_XCD;
+    if (lb >= nblk) return;  // (the kernel has no barrier)
+    const int bx = lb % gx, by = (lb / gx) % gy, slot = lb / (gx * gy);
     const int k = dense_k[slot];
-    const int r0 = blockIdx.y * MF_BM, c0 = blockIdx.x * MF_BN;
-    const c128* W = WT + (size_t)slot * N * N;
-    const cplx<Tg>* Y = reinterpret_cast<const cplx<Tg>*>(Yin);
-    cplx<Tg>* Z = reinterpret_cast<cplx<Tg>*>(Zout);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    v4d cre[2][2], cim[2][2];
+    const int r0 = by * 64 + wave * 16;
+    const int c0 = bx * 16;
+    if (r0 >= SP) return;
+    const int li = lane & 15, kq = lane >> 4;
+    const c128* W = WT + (size_t)slot * N * N + c0 + li;  // B[kq][li] of k-step kb: W[(kb + kq) * N]
+    const int row = r0 + li;
+    const bool rv = row < SP;
+    const cplx<Tg>* Arow = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv ? row : 0) * L + k) * N + kq;
+    v4d cre = v4d{0.0, 0.0, 0.0, 0.0}, cim = v4d{0.0, 0.0, 0.0, 0.0};
+    c128 a[MF_U], b[MF_U], an[MF_U], bn[MF_U];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) cre[a][b] = cim[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int j0 = 0; j0 < N; j0 += MF_KC) {
-        for (int idx = threadIdx.x; idx < MF_BM * MF_KC; idx += 256) {
-            const int rr = idx / MF_KC, jj = idx % MF_KC;
-            const int sp = r0 + rr;
-            As[rr][jj] = sp < SP ? cast<double>(Y[((size_t)sp * L + k) * N + j0 + jj]) : mk(0.0, 0.0);
-        }
-        for (int idx = threadIdx.x; idx < MF_KC * MF_BN; idx += 256) {
-            const int jj = idx / MF_BN, cc = idx % MF_BN;
-            Bs[jj][cc] = (c0 + cc < N) ? W[(size_t)(j0 + jj) * N + c0 + cc] : mk(0.0, 0.0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < MF_KC; kk += 4) {
-            double are[2], aim[2], bre[2], bim[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const c128 v = As[wr + a * 16 + (lane & 15)][kk + (lane >> 4)];
-                are[a] = v.x;
-                aim[a] = v.y;
-            }
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const c128 v = Bs[kk + (lane >> 4)][wc + b * 16 + (lane & 15)];
-                bre[b] = v.x;
-                bim[b] = v.y;
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[a], bre[b], cre[a][b], 0, 0, 0);
-                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[a], bim[b], cre[a][b], 0, 0, 0);
-                    cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[a], bim[b], cim[a][b], 0, 0, 0);
-                    cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[a], bre[b], cim[a][b], 0, 0, 0);
-                }
-        }
-        __syncthreads();
+    for (int u = 0; u < MF_U; ++u) {
+        a[u] = rv ? cast<double>(Arow[4 * u]) : mk(0.0, 0.0);
+        b[u] = W[(size_t)(4 * u + kq) * N];
     }
+    for (int kb = 0; kb < N; kb += 4 * MF_U) {
+        const int kn = kb + 4 * MF_U;
+        if (kn < N) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int sp = r0 + wr + a * 16 + 4 * e + (lane >> 4);
-                const int i = c0 + wc + b * 16 + (lane & 15);
-                if (sp < SP && i < N) Z[((size_t)sp * L + k) * N + i] = cast<Tg>(mk(cre[a][b][e], cim[a][b][e]));
+            for (int u = 0; u < MF_U; ++u) {
+                an[u] = rv ? cast<double>(Arow[kn + 4 * u]) : mk(0.0, 0.0);
+                bn[u] = W[(size_t)(kn + 4 * u + kq) * N];
             }
+        }
+#pragma unroll
+        for (int u = 0; u < MF_U; ++u) {
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, b[u].x, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[u].y, b[u].y, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, b[u].y, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, b[u].x, cim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < MF_U; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
+    }
+    cplx<Tg>* Z = reinterpret_cast<cplx<Tg>*>(Zout);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int sp = r0 + 4 * e + kq;
+        if (sp < SP) Z[((size_t)sp * L + k) * N + c0 + li] = cast<Tg>(mk(cre[e], cim[e]));
+    }
 }
 
 // copies the dense rows of Y2 back into Y
@@ -198,8 +194,12 @@ int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c12
                 void* tmp, hipStream_t st) {
     const int N = 1 << n;
     void* out = (Z == Y) ? tmp : Z;
-    hipLaunchKernelGGL(dense_mfma_kernel<Tg>, dim3((N + MF_BN - 1) / MF_BN, (SP + MF_BM - 1) / MF_BM, n_dense),
-                       dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out);
+    const int gx = N / 16, gy = (SP + 63) / 64;
+    const long long nblk = (long long)gx * gy * n_dense;
+    ADRT_REQUIRE(nblk <= 0x7fffffffLL, "dense_mfma: too many blocks");
+    const int per = (int)((nblk + MF_XCD - 1) / MF_XCD);
+    hipLaunchKernelGGL(dense_mfma_kernel<Tg>, dim3(per * MF_XCD), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out, gx, gy,
+                       (int)nblk);
     ADRT_TRY(launch_check("dense_mfma_kernel"));
     if (out != Z) {
         long long cnt = (long long)SP * N;

@@ -1,5 +1,7 @@
 // Image-space kernels of the outer iteration (fastops.py:247-258) and the level-1 preconditioners
 // (fasthybrid.py:81-130): Chebyshev updates, the G_1 stencil step, the lorr projector.  All in double.
+#include <type_traits>
+
 #include "passes.hpp"
 
 namespace adrt {
@@ -74,11 +76,16 @@ __global__ void lorr_kernel(int n, const double* y, double* out, long long count
 //
 // Each thread owns 2 x 2 cells (up to BPT of them) and keeps their y, current and previous iterate in registers.  For
 // a cell (i, j), G_1 x = 8 x + [1 2 1] down the partner column j^1 (rows i-1..i+1) + [1 2 1] along the partner row i^1
-// (columns j-1..j+1) (fasthybrid.py:81-98); inside a 2 x 2 cell both partners are in the cell, so a step reads only
-// eight neighbours from LDS (rows above / below, columns left / right of the cell) instead of seven per pixel.  The
-// iterate is double-buffered in LDS (step k reads buffer (k-1)&1 and writes k&1), one barrier per step.  Pixels
-// outside the image and a one-pixel border around the region hold zero, the stencil's zero extension.  The epilogue
-// optionally scales (mode 1) or applies the outer Chebyshev update (mode 2), so P and the outer step cost one pass.
+// (columns j-1..j+1) (fasthybrid.py:81-98); inside a 2 x 2 cell both partners are in the cell, so a step needs only
+// the four neighbouring cells: the bottom row of the cell above, the top row of the cell below, the right column of
+// the cell to the left and the left column of the cell to the right.  LDS holds the iterate cell-major, one 4-vector
+// per cell (x00, x01, x10, x11), so each neighbour is one aligned 16 / 32-byte read, double-buffered (step k reads
+// buffer (k-1)&1 and writes k&1, one barrier per step).  Cells are numbered ring by ring: the tile's cells first,
+// then the rings of halo cells at Chebyshev distance 1, 2, ... cells, and cell b belongs to thread b % THREADS.  The
+// cells a step must compute (rings up to (deg - k) / 2) are then a prefix of that numbering, so whole waves drop out
+// as the square shrinks.  Cells outside the image and a one-cell border around the region hold zero, the stencil's
+// zero extension.  The epilogue optionally scales (mode 1) or applies the outer Chebyshev update (mode 2), so P and
+// the outer step cost one pass.
 constexpr int G1_MAXH = 16;
 constexpr int G1_THREADS = 512;
 
@@ -92,83 +99,125 @@ struct G1Args {
     double om[32];     // inner omega_k, k = 1..deg-1
 };
 
-template <int TILE, int HALO>
+template <typename T>
+struct alignas(4 * sizeof(T)) Cell4 {
+    T v[4];  // x00, x01, x10, x11
+};
+
+template <typename T, int TILE, int HALO>
 struct G1Cfg {
     static constexpr int R = TILE + 2 * HALO;  // region side (even)
     static constexpr int RB = R / 2;           // 2 x 2 cells per region side
-    static constexpr int RP = R + 2;           // LDS pitch: region plus a zero border on each side
-    static constexpr int BUF = RP * RP;
+    static constexpr int TB = TILE / 2;        // cells per tile side
+    static constexpr int HB = HALO / 2;        // halo rings (cells)
+    static constexpr int CP = RB + 2;          // LDS pitch in cells: the region plus a zero border cell on each side
+    static constexpr int BUF = CP * CP;
     static constexpr int NB = RB * RB;
     static constexpr int BPT = (NB + G1_THREADS - 1) / G1_THREADS;
-    static constexpr size_t LDS = 2 * BUF * sizeof(double);
+    static constexpr size_t LDS = 2 * (size_t)BUF * sizeof(Cell4<T>);
+    // number of cells in the tile and its first j rings
+    __host__ __device__ static constexpr int upto(int j) { return (TB + 2 * j) * (TB + 2 * j); }
 };
 
-template <int TILE, int HALO>
+// Region cell (row, column) of ring-ordered cell b and its ring (0 = inside the tile).
+template <int TB, int HB>
+__device__ __forceinline__ void ring_cell(int b, int& bi, int& bj, int& ring) {
+    if (b < TB * TB) {
+        bi = HB + b / TB;
+        bj = HB + b % TB;
+        ring = 0;
+        return;
+    }
+    int j = 1, r = b - TB * TB;
+    while (j < HB && r >= 4 * (TB + 2 * j - 1)) {
+        r -= 4 * (TB + 2 * j - 1);
+        ++j;
+    }
+    const int side = TB + 2 * j, o = HB - j;  // the ring is the border of a side x side square at (o, o)
+    if (r < side) {
+        bi = o;
+        bj = o + r;
+    } else if (r < 2 * side) {
+        bi = o + side - 1;
+        bj = o + r - side;
+    } else {
+        const int q = r - 2 * side;  // 2 (side - 2) cells of the left and right columns
+        bi = o + 1 + (q >> 1);
+        bj = (q & 1) ? o + side - 1 : o;
+    }
+    ring = j;
+}
+
+// T is the type of the inner iteration (double, or float inside the FP32 outer steps of an FP32 plan); the
+// right-hand side, the outer iterates and the epilogue are double.
+template <typename T, int TILE, int HALO>
 __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
-    using C = G1Cfg<TILE, HALO>;
-    constexpr int RB = C::RB, RP = C::RP, BUF = C::BUF, NB = C::NB, BPT = C::BPT;
-    extern __shared__ double X[];  // two buffers of BUF doubles
+    using C = G1Cfg<T, TILE, HALO>;
+    constexpr int CP = C::CP, BUF = C::BUF, NB = C::NB, BPT = C::BPT, TB = C::TB, HB = C::HB;
+    extern __shared__ __align__(16) unsigned char g1_smem[];
+    Cell4<T>* X = reinterpret_cast<Cell4<T>*>(g1_smem);  // two buffers of (CP, CP) cells
     const int N = 1 << A.n;
     const int H = A.deg - 1;  // <= HALO
     const int s = blockIdx.z;
     const int gi0 = blockIdx.y * TILE - HALO, gj0 = blockIdx.x * TILE - HALO;
     const double* ys = A.y + (size_t)s * N * N;
-    const double a = 2.0 / 20.0;
-    for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = 0.0;
+    const T a = (T)(2.0 / 20.0);
+    for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = Cell4<T>{{(T)0, (T)0, (T)0, (T)0}};
     __syncthreads();
-    double yv[BPT][4], xc[BPT][4], xq[BPT][4];
+    T yv[BPT][4], xc[BPT][4], xq[BPT][4];
     unsigned inm[BPT];
+    int lds[BPT];  // LDS cell index of the thread's cells
 #pragma unroll
     for (int e = 0; e < BPT; ++e) {
         const int b = threadIdx.x + G1_THREADS * e;
-        const int bi = b / RB, bj = b % RB;
+        int bi = 0, bj = 0, ring = 0;
+        if (b < NB) ring_cell<TB, HB>(b, bi, bj, ring);
+        lds[e] = (bi + 1) * CP + bj + 1;
         inm[e] = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int ri = 2 * bi + (c >> 1), rj = 2 * bj + (c & 1);
-            const int gi = gi0 + ri, gj = gj0 + rj;
+            const int gi = gi0 + 2 * bi + (c >> 1), gj = gj0 + 2 * bj + (c & 1);
             const bool in = b < NB && gi >= 0 && gi < N && gj >= 0 && gj < N;
             inm[e] |= (in ? 1u : 0u) << c;
-            yv[e][c] = in ? ys[(size_t)gi * N + gj] : 0.0;
+            yv[e][c] = in ? (T)ys[(size_t)gi * N + gj] : (T)0;
             xc[e][c] = a * yv[e][c];
-            xq[e][c] = 0.0;
-            if (b < NB) X[(ri + 1) * RP + rj + 1] = xc[e][c];
+            xq[e][c] = (T)0;
         }
+        if (b < NB) X[lds[e]] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
     }
     __syncthreads();
     for (int k = 1; k <= H; ++k) {
-        const double om = A.om[k - 1], om1 = om - 1.0;
-        const double* Xr = X + ((k - 1) & 1) * BUF;
-        double* Xw = X + (k & 1) * BUF;
-        // region rows / columns that must be exact after step k
-        const int lo = HALO - (H - k), hi = HALO + TILE + (H - k);
+        const T om = (T)A.om[k - 1], om1 = (T)(A.om[k - 1] - 1.0);
+        const Cell4<T>* Xr = X + ((k - 1) & 1) * BUF;
+        Cell4<T>* Xw = X + (k & 1) * BUF;
+        // the cells that must be exact after step k: rings j with 2 j - 1 <= H - k, a prefix of the numbering
+        const int jmax = min((H - k + 1) / 2, HB);
+        const int active = C::upto(jmax);
 #pragma unroll
         for (int e = 0; e < BPT; ++e) {
             const int b = threadIdx.x + G1_THREADS * e;
-            const int r0 = 2 * (b / RB), c0 = 2 * (b % RB);
-            if (b >= NB || r0 + 1 < lo || r0 >= hi || c0 + 1 < lo || c0 >= hi) continue;
-            const int o = (r0 + 1) * RP + c0 + 1;  // LDS index of cell corner (r0, c0)
-            const double up0 = Xr[o - RP], up1 = Xr[o - RP + 1];
-            const double dn0 = Xr[o + 2 * RP], dn1 = Xr[o + 2 * RP + 1];
-            const double lf0 = Xr[o - 1], lf1 = Xr[o + RP - 1];
-            const double rt0 = Xr[o + 2], rt1 = Xr[o + RP + 2];
-            const double x00 = xc[e][0], x01 = xc[e][1], x10 = xc[e][2], x11 = xc[e][3];
-            double g[4];
-            g[0] = 8.0 * x00 + (up1 + 2.0 * x01 + x11) + (lf1 + 2.0 * x10 + x11);
-            g[1] = 8.0 * x01 + (up0 + 2.0 * x00 + x10) + (x10 + 2.0 * x11 + rt1);
-            g[2] = 8.0 * x10 + (x01 + 2.0 * x11 + dn1) + (lf0 + 2.0 * x00 + x01);
-            g[3] = 8.0 * x11 + (x00 + 2.0 * x10 + dn0) + (x00 + 2.0 * x01 + rt0);
+            if (b >= active) continue;
+            const int o = lds[e];
+            const Cell4<T> up = Xr[o - CP], dn = Xr[o + CP], lf = Xr[o - 1], rt = Xr[o + 1];
+            const T up0 = up.v[2], up1 = up.v[3];  // x[r0 - 1][c0], x[r0 - 1][c0 + 1]
+            const T dn0 = dn.v[0], dn1 = dn.v[1];  // x[r0 + 2][c0], x[r0 + 2][c0 + 1]
+            const T lf0 = lf.v[1], lf1 = lf.v[3];  // x[r0][c0 - 1], x[r0 + 1][c0 - 1]
+            const T rt0 = rt.v[0], rt1 = rt.v[2];  // x[r0][c0 + 2], x[r0 + 1][c0 + 2]
+            const T x00 = xc[e][0], x01 = xc[e][1], x10 = xc[e][2], x11 = xc[e][3];
+            const T two = (T)2, eight = (T)8;
+            T g[4];
+            g[0] = eight * x00 + (up1 + two * x01 + x11) + (lf1 + two * x10 + x11);
+            g[1] = eight * x01 + (up0 + two * x00 + x10) + (x10 + two * x11 + rt1);
+            g[2] = eight * x10 + (x01 + two * x11 + dn1) + (lf0 + two * x00 + x01);
+            g[3] = eight * x11 + (x00 + two * x10 + dn0) + (x00 + two * x01 + rt0);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const double xo = xc[e][c];
-                const double xn = ((inm[e] >> c) & 1) ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : 0.0;
+                const T xo = xc[e][c];
+                const T xn = ((inm[e] >> c) & 1) ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : (T)0;
                 xq[e][c] = xo;
                 xc[e][c] = xn;
             }
-            Xw[o] = xc[e][0];
-            Xw[o + 1] = xc[e][1];
-            Xw[o + RP] = xc[e][2];
-            Xw[o + RP + 1] = xc[e][3];
+            Xw[o] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
         }
         __syncthreads();
     }
@@ -176,15 +225,14 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
 #pragma unroll
     for (int e = 0; e < BPT; ++e) {
         const int b = threadIdx.x + G1_THREADS * e;
-        if (b >= NB) continue;
-        const int bi = b / RB, bj = b % RB;
+        if (b >= TB * TB) continue;  // the tile's cells come first
+        const int bi = HB + b / TB, bj = HB + b % TB;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int ri = 2 * bi + (c >> 1), rj = 2 * bj + (c & 1);
-            const int gi = gi0 + ri, gj = gj0 + rj;
-            if (ri < HALO || rj < HALO || ri >= HALO + TILE || rj >= HALO + TILE || gi >= N || gj >= N) continue;
+            const int gi = gi0 + 2 * bi + (c >> 1), gj = gj0 + 2 * bj + (c & 1);
+            if (gi >= N || gj >= N) continue;
             const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
-            const double pv = xc[e][c];
+            const double pv = (double)xc[e][c];
             if (A.mode == 0) {
                 A.out[o] = pv;
             } else if (A.mode == 1) {
@@ -198,19 +246,30 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     }
 }
 
-template <int TILE, int HALO>
+template <typename T, int TILE, int HALO>
 int launch_g1_cfg(const G1Args& A, int S, hipStream_t st) {
     const int N = 1 << A.n;
-    const int T = (N + TILE - 1) / TILE;
-    constexpr size_t lds = G1Cfg<TILE, HALO>::LDS;
-    hipLaunchKernelGGL((g1_fused_kernel<TILE, HALO>), dim3(T, T, S), dim3(G1_THREADS), lds, st, A);
+    const int nt = (N + TILE - 1) / TILE;
+    constexpr size_t lds = G1Cfg<T, TILE, HALO>::LDS;
+    hipLaunchKernelGGL((g1_fused_kernel<T, TILE, HALO>), dim3(nt, nt, S), dim3(G1_THREADS), lds, st, A);
     return launch_check("g1_fused_kernel");
+}
+
+template <typename T>
+int launch_g1_t(const G1Args& A, int S, hipStream_t st) {
+    const int N = 1 << A.n;
+    // TILE 64 halves the halo overhead; TILE 32 gives a small batch more blocks.  The float iteration always takes
+    // TILE 64: a pixel's arithmetic then does not depend on the batch size, so a sinogram gives the same bits alone
+    // and inside a batch (in double the tiles differ by at most a rounding of the last bit).
+    const bool big = std::is_same<T, float>::value || (long long)S * ((N + 63) / 64) * ((N + 63) / 64) >= 256;
+    if (A.deg - 1 <= 14) return big ? launch_g1_cfg<T, 64, 14>(A, S, st) : launch_g1_cfg<T, 32, 14>(A, S, st);
+    return big ? launch_g1_cfg<T, 64, 16>(A, S, st) : launch_g1_cfg<T, 32, 16>(A, S, st);
 }
 
 }  // namespace
 
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
-                    double w1, double w0, double* out, int S, hipStream_t st) {
+                    double w1, double w0, double* out, int S, hipStream_t st, int prec) {
     ADRT_REQUIRE(deg >= 1 && deg - 1 <= G1_MAXH, "g1_fused: g1_deg must be 1..17");
     ADRT_REQUIRE(n >= 1 && n <= 13, "g1_fused: need 1 <= n <= 13");
     if (S <= 0) return OK;
@@ -231,11 +290,7 @@ int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, 
         om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
         A.om[k - 1] = om;
     }
-    const int N = 1 << n;
-    // TILE 64 halves the halo overhead; TILE 32 gives a small batch more blocks
-    const bool big = (long long)S * ((N + 63) / 64) * ((N + 63) / 64) >= 256;
-    if (deg - 1 <= 14) return big ? launch_g1_cfg<64, 14>(A, S, st) : launch_g1_cfg<32, 14>(A, S, st);
-    return big ? launch_g1_cfg<64, 16>(A, S, st) : launch_g1_cfg<32, 16>(A, S, st);
+    return prec == PREC_F32 ? launch_g1_t<float>(A, S, st) : launch_g1_t<double>(A, S, st);
 }
 
 int launch_scale(const double* x, double alpha, double* out, long long count, hipStream_t st) {

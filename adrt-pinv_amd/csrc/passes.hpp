@@ -37,8 +37,9 @@ int launch_g1_step(int n, const double* y, const double* x, const double* xp, do
                    double* out, int S, hipStream_t st);
 // g1_inv(y) (deg Chebyshev steps of G_1 on [4, 16]) in one launch, with an epilogue:
 //   mode 0: out = P y;  mode 1: out = a_out * P y;  mode 2: out = x + w1 * (a_out * P y) + w0 * (x - xp) (xp null = 0)
+// prec PREC_F32 runs the inner iteration in float (inputs, outer iterates and the epilogue stay double).
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
-                    double w1, double w0, double* out, int S, hipStream_t st);
+                    double w1, double w0, double* out, int S, hipStream_t st, int prec = PREC_F64);
 // out = (y - Pi y)/4 + Pi y / 12 (fasthybrid.py:101-111)
 int launch_lorr(int n, const double* y, double* out, int S, hipStream_t st);
 

@@ -30,7 +30,7 @@ struct Plan {
     void* tw32 = nullptr;  // the same rounded to float
     // scratch for max_batch sinograms
     void* Y = nullptr;   // (S, 2, L, N) complex rows
-    void* Y2 = nullptr;  // second row buffer, allocated on first use by adrt_plan_rows_pass
+    void* Y2 = nullptr;  // (S, 2, L, N) output rows of the row pass (out of place: the dense GEMM reads Y whole)
     void* Sq = nullptr;  // (S, 4, N, N) real quadrant strips
     double* y = nullptr;
     double* xs = nullptr;
@@ -93,8 +93,11 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
             ADRT_TRY(launch_rows(A, prec, st));
             if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
         }
-        if (p->d.n >= 4 && 2 * S > 4) {  // the MFMA tile (64 rows) pays off once several sinograms share W
-            if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
+        // The MFMA tile (64 rows) pays off once several sinograms share W.  Every pass of an FP32 plan takes it (its
+        // FP64 passes too): its per-row arithmetic does not depend on the batch size, so a sinogram gives the same
+        // bits alone and in a batch.  (A last-bit difference of an FP64 pass would otherwise reach an FP32 rounding
+        // of the residual, whose cancellation magnifies it, and grow to the FP32 level.)
+        if (p->d.n >= 4 && (p->d.precision == PREC_F32 || 2 * S > 4)) {
             return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, p->Y2, prec, st);
         }
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
@@ -112,8 +115,8 @@ int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops
     const int n = p->d.n;
     const void* tw = prec == PREC_F64 ? p->tw : p->tw32;
     ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, tw, st));
-    ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, nullptr, mode == 1 ? p->WbackT : p->WopT));
-    ADRT_TRY(launch_ifft_unpack(n, p->Y, p->Sq, S, prec, tw, st));
+    ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, p->Y2, mode == 1 ? p->WbackT : p->WopT));
+    ADRT_TRY(launch_ifft_unpack(n, p->Y2, p->Sq, S, prec, tw, st));
     return launch_combine(n, p->Sq, prec, resid_y, out, S, st);
 }
 
@@ -167,7 +170,7 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
     if (d->dense_on_device != 2)
         b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
     b += twiddle_bytes(d->n, PREC_F64) + twiddle_bytes(d->n, PREC_F32);
-    b += S * 2 * L * N * row_elem(PREC_F64);
+    b += 2 * S * 2 * L * N * row_elem(PREC_F64);
     b += S * 4 * N * N * strip_elem(PREC_F64);
     b += 5 * S * N * N * sizeof(double);
     return b;
@@ -323,6 +326,7 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     }
     const size_t S = d->max_batch, NN = (size_t)N * N;
     if ((st = alloc(&p->Y, S * 2 * L * N * row_elem(PREC_F64)))) return fail(st);
+    if ((st = alloc(&p->Y2, S * 2 * L * N * row_elem(PREC_F64)))) return fail(st);
     if ((st = alloc(&p->Sq, S * 4 * NN * strip_elem(PREC_F64)))) return fail(st);
     if ((st = alloc((void**)&p->y, S * NN * sizeof(double)))) return fail(st);
     if ((st = alloc((void**)&p->xs, S * NN * sizeof(double)))) return fail(st);
@@ -400,7 +404,6 @@ int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream) {
     ADRT_TRY(check_batch(p, S));
     ADRT_REQUIRE(which == 0 || which == 1, "plan_rows_pass: which must be 0 (backproject ops) or 1 (operator ops)");
     if (S == 0) return OK;
-    if (!p->Y2) ADRT_TRY(adrt_malloc(&p->Y2, (size_t)p->d.max_batch * 2 * p->L * p->N * row_elem(PREC_F64)));
     return which == 0 ? rows_pass(p, p->d.precision, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2, p->WbackT)
                       : rows_pass(p, p->d.precision, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2, p->WopT);
 }
@@ -438,7 +441,8 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     const bool fused = has_p && p->d.precond == ADRT_PRECOND_G1 && p->d.g1_deg - 1 <= 16;
     const double* py = p->y;
     if (fused) {
-        ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, p->y, nullptr, nullptr, a, 0.0, 0.0, X[0], S, st));
+        ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, p->y, nullptr, nullptr, a, 0.0, 0.0, X[0], S, st,
+                                 refine > 0 ? PREC_F64 : prec));
     } else {
         if (has_p) {
             ADRT_TRY(precond(p, p->y, p->r, S, st));
@@ -454,7 +458,7 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
         const int pk = (refine > 0 && k >= D - refine) ? PREC_F64 : prec;
         ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st));
         if (fused) {
-            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, p->res, xk, xp, a, om, om - 1.0, X[k % 2], S, st));
+            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, p->res, xk, xp, a, om, om - 1.0, X[k % 2], S, st, pk));
             continue;
         }
         const double* rr = p->res;

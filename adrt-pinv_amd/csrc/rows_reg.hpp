@@ -1,0 +1,452 @@
+// Register-resident row kernel (templates; instantiated per compute type and lower level by rows_reg_*.hip).
+//
+// A row of N = 2^n complex values is held by G = N / E lanes, E = 2^e values per lane, RPB = max(64, G) / G rows per
+// block.  The n butterfly bits are split into passes of e consecutive bits ([0, e), [e, 2e), ...).  In pass K a lane
+// holds the E values whose physical indices differ only in the pass's bits, so every level of the pass is a
+// register-local radix-2 step (in-place scheme of rows.hpp: pair bit m-1, twiddle z^t with t = bitrev_{m-1} of the low
+// bits).  Between passes the row goes through LDS once (store in one grouping, load in the next).  A Chebyshev step is
+// forward passes 0..K, adjoint passes K..0, so the row leaves and re-enters pass 0's grouping, which is the natural
+// order: the iterate z, its predecessor and y stay in registers across all steps.
+//
+// The level range of every op is a compile-time constant: levels LO..n for the Chebyshev filter, the Gram sandwich
+// and the end-of-list forward / adjoint ops (LO = 1 for L_p, 2 for the hybrid), level 1 alone for the explicit level.
+// Those are the only ranges the inverses use (fastops.py:281-285, 330-335), so the passes are straight-line code.
+// Each lane keeps the pair (z^t, z^(t+1)) of every twiddle slot it uses in registers: the forward pair is
+// (f0 + z^t f1, f0 + z^(t+1) f1), the adjoint (g0 + g1, conj(z^t) g0 + conj(z^(t+1)) g1), two FMAs per real component,
+// and no twiddle product is recomputed inside the Chebyshev loop.  The twiddles come from a table of z^t, t <= N/2,
+// that the row's lanes build in LDS at start (double products of z^(2^b), rounded to the compute type).
+//
+// Rows of one block run the same macro-op program; P rows with different degrees iterate to the block's largest
+// degree with the update predicated off, so every barrier is block-uniform.  The plan maps blocks to rows of one
+// frequency (equal degree) in descending-degree order.
+#pragma once
+#include "rows.hpp"
+
+namespace adrt {
+
+// macro-ops decoded on the host from the reference op list (rows_reg.hip: decode_ops)
+enum MacroOp : int { MOP_L1F = 0, MOP_L1A = 1, MOP_P = 2, MOP_GRAM = 3, MOP_BTNAT = 4, MOP_BEND = 5 };
+
+struct RegArgs {
+    RowsArgs r;
+    int nmops;
+    int mop[MAX_OPS];
+    int lo;              // lower level of every P / GRAM / BTNAT / BEND macro-op (1 or 2); their upper level is n
+    int group_by_k;      // blocks own rows of one frequency (rows = SP * period)
+    int SP;              // rows per frequency when group_by_k
+    const int* korder;   // frequency order of the blocks (may be null)
+    int nk;              // frequencies scheduled (korder entries)
+};
+
+// Launchers of one compute type and lower level, rows_reg_<type><lo>.hip (d: double, s: float, c: float with the
+// TwoSum-compensated Chebyshev iterate); E_INVALID for an unsupported size.
+int rows_reg_launch_d1(const RegArgs& RA, int loge, hipStream_t s);
+int rows_reg_launch_d2(const RegArgs& RA, int loge, hipStream_t s);
+int rows_reg_launch_s1(const RegArgs& RA, int loge, hipStream_t s);
+int rows_reg_launch_s2(const RegArgs& RA, int loge, hipStream_t s);
+int rows_reg_launch_c1(const RegArgs& RA, int loge, hipStream_t s);
+int rows_reg_launch_c2(const RegArgs& RA, int loge, hipStream_t s);
+
+namespace rreg {
+
+__device__ __forceinline__ int brev_low(int x, int bits) {
+    return bits <= 0 ? 0 : (int)(__brev((unsigned)x) >> (32 - bits));
+}
+
+// physical index held after forward levels 1..hi for natural index x, and vice versa (bitrev of the low hi bits)
+__device__ __forceinline__ int lam(int x, int hi) {
+    if (hi <= 1) return x;
+    const int mask = (1 << hi) - 1;
+    return (x & ~mask) | brev_low(x & mask, hi);
+}
+
+template <typename T, int LOGN, int LOGE>
+struct Cfg {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int E = 1 << LOGE;
+    static constexpr int G = N / E;
+    static constexpr int NT = G < 64 ? 64 : G;
+    static constexpr int RPB = NT / G;
+    static constexpr int NPASS = (LOGN + LOGE - 1) / LOGE;
+    static constexpr int ROW = N + (N >> LOGE);  // padded row length in LDS (also holds the z^t table, N/2 + 1)
+    __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
+    __device__ static __forceinline__ int pad(int p) { return p + (p >> LOGE); }
+    // physical index of lane-local element u in the grouping of pass K
+    template <int K>
+    __device__ static __forceinline__ int phys(int lane, int u) {
+        constexpr int B0 = K * LOGE;
+        constexpr int KB = kb(K);
+        const int j = u & ((1 << KB) - 1);
+        const int gs = u >> KB;
+        const int q = lane + G * gs;
+        return ((q >> B0) << (B0 + KB)) | (j << B0) | (q & ((1 << B0) - 1));
+    }
+};
+
+// ---- one pass of levels LO..HI (compile-time) forward / adjoint, in registers.  Slot (gs, i, jl) of pass K holds
+// the twiddle pair of the lane's pairs that share a group gs and low bits jl on the pass's level i.
+template <typename T, int LOGN, int LOGE, int K, int LO, int HI>
+__device__ __forceinline__ void pass_fwd(cplx<T> (&v)[1 << LOGE], const cplx<T>* tw, const cplx<T>* tz) {
+    using C = Cfg<T, LOGN, LOGE>;
+    constexpr int KB = C::kb(K);
+    constexpr int B0 = K * LOGE;
+    constexpr int PER = (1 << KB) - 1;
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+        const int m = B0 + i + 1;
+        if (m < LO || m > HI) continue;
+#pragma unroll
+        for (int gs = 0; gs < (C::E >> KB); ++gs) {
+#pragma unroll
+            for (int jl = 0; jl < (1 << i); ++jl) {
+                const bool one = B0 == 0 && jl == 0;  // t = 0: z^t = 1
+                const int slot = gs * PER + (1 << i) - 1 + jl;
+#pragma unroll
+                for (int h = 0; h < (1 << (KB - 1 - i)); ++h) {
+                    const int u0 = (gs << KB) | (h << (i + 1)) | jl;
+                    const int u1 = u0 | (1 << i);
+                    const cplx<T> f0 = v[u0], f1 = v[u1];
+                    v[u0] = one ? add(f0, f1) : cfma(tw[slot], f1, f0);
+                    v[u1] = cfma(tz[slot], f1, f0);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int LOGN, int LOGE, int K, int LO, int HI>
+__device__ __forceinline__ void pass_adj(cplx<T> (&v)[1 << LOGE], const cplx<T>* tw, const cplx<T>* tz) {
+    using C = Cfg<T, LOGN, LOGE>;
+    constexpr int KB = C::kb(K);
+    constexpr int B0 = K * LOGE;
+    constexpr int PER = (1 << KB) - 1;
+#pragma unroll
+    for (int i = KB - 1; i >= 0; --i) {
+        const int m = B0 + i + 1;
+        if (m < LO || m > HI) continue;
+#pragma unroll
+        for (int gs = 0; gs < (C::E >> KB); ++gs) {
+#pragma unroll
+            for (int jl = 0; jl < (1 << i); ++jl) {
+                const bool one = B0 == 0 && jl == 0;
+                const int slot = gs * PER + (1 << i) - 1 + jl;
+#pragma unroll
+                for (int h = 0; h < (1 << (KB - 1 - i)); ++h) {
+                    const int u0 = (gs << KB) | (h << (i + 1)) | jl;
+                    const int u1 = u0 | (1 << i);
+                    const cplx<T> g0 = v[u0], g1 = v[u1];
+                    v[u0] = add(g0, g1);
+                    v[u1] = one ? cfmac(tz[slot], g1, g0) : cfmac(tz[slot], g1, cmulc(tw[slot], g0));
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int LOGN, int LOGE, int KA, int KB_>
+__device__ __forceinline__ void transition(cplx<T> (&v)[1 << LOGE], cplx<T>* X, int lane) {
+    using C = Cfg<T, LOGN, LOGE>;
+#pragma unroll
+    for (int u = 0; u < C::E; ++u) X[C::pad(C::template phys<KA>(lane, u))] = v[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < C::E; ++u) v[u] = X[C::pad(C::template phys<KB_>(lane, u))];
+    __syncthreads();
+}
+
+// levels LO..n forward then adjoint (the Gram B^H B), grouping 0 -> grouping 0
+template <typename T, int LOGN, int LOGE, int K, int LO>
+__device__ __forceinline__ void gram_rec(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
+                                         const cplx<T> (*tz)[1 << LOGE], int lane) {
+    using C = Cfg<T, LOGN, LOGE>;
+    pass_fwd<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
+    if constexpr (K + 1 < C::NPASS) {
+        transition<T, LOGN, LOGE, K, K + 1>(v, X, lane);
+        gram_rec<T, LOGN, LOGE, K + 1, LO>(v, X, tw, tz, lane);
+        transition<T, LOGN, LOGE, K + 1, K>(v, X, lane);
+    }
+    pass_adj<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
+}
+
+// adjoint passes K..0 of levels LO..n starting from grouping K
+template <typename T, int LOGN, int LOGE, int K, int LO>
+__device__ __forceinline__ void adj_down(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
+                                         const cplx<T> (*tz)[1 << LOGE], int lane) {
+    pass_adj<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
+    if constexpr (K > 0) {
+        transition<T, LOGN, LOGE, K, K - 1>(v, X, lane);
+        adj_down<T, LOGN, LOGE, K - 1, LO>(v, X, tw, tz, lane);
+    }
+}
+
+// forward passes K..NPASS-1 of levels LO..n starting from grouping K, ending in grouping NPASS-1
+template <typename T, int LOGN, int LOGE, int K, int LO>
+__device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
+                                       const cplx<T> (*tz)[1 << LOGE], int lane) {
+    pass_fwd<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
+    if constexpr (K + 1 < Cfg<T, LOGN, LOGE>::NPASS) {
+        transition<T, LOGN, LOGE, K, K + 1>(v, X, lane);
+        fwd_up<T, LOGN, LOGE, K + 1, LO>(v, X, tw, tz, lane);
+    }
+}
+
+template <typename T, int LOGN, int LOGE, bool COMP, int LO>
+__global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegArgs RA) {
+    using C = Cfg<T, LOGN, LOGE>;
+    constexpr int E = C::E;
+    constexpr int NP = C::NPASS;
+    constexpr int LAST = NP - 1;
+    const RowsArgs& A = RA.r;
+    __shared__ cplx<T> Xs[C::RPB * C::ROW];
+    __shared__ int sdeg[C::RPB];
+
+    const int tid = threadIdx.x;
+    const int rib = tid / C::G;   // row in block
+    const int lane = tid % C::G;  // lane within the row
+    cplx<T>* X = Xs + rib * C::ROW;
+
+    // ---- which row
+    long long row;
+    bool valid;
+    if (RA.group_by_k) {
+        const int nspb = (RA.SP + C::RPB - 1) / C::RPB;
+        const int kk = blockIdx.x / nspb;
+        const int sg = blockIdx.x % nspb;
+        const int k = RA.korder ? RA.korder[kk] : kk;
+        const int sp = sg * C::RPB + rib;
+        valid = sp < RA.SP;
+        row = (long long)(valid ? sp : 0) * A.period + k;
+    } else {
+        row = (long long)blockIdx.x * C::RPB + rib;
+        valid = row < A.rows;
+        if (!valid) row = 0;
+    }
+    const int k = (int)(row % A.period);
+    const c128 zk = A.z[k];
+
+    // ---- table of z^t, t = 0 .. N/2, in the row's LDS buffer: products of z^(2^b) in double, rounded to T
+    {
+        c128 p2[LOGN];
+        c128 p = zk;
+#pragma unroll
+        for (int j = 0; j < LOGN; ++j) {
+            p2[j] = p;
+            p = mul(p, p);
+        }
+        for (int t = lane; t <= C::N / 2; t += C::G) {
+            c128 w = mk(1.0, 0.0);
+#pragma unroll
+            for (int b = 0; b < LOGN; ++b)
+                if ((t >> b) & 1) w = mul(w, p2[b]);
+            X[t] = cast<T>(w);
+        }
+    }
+    __syncthreads();
+    // ---- the lane's twiddle pairs (z^t, z^(t+1)) of every pass
+    cplx<T> tw[NP][E], tz[NP][E];
+#pragma unroll
+    for (int K = 0; K < NP; ++K) {
+        const int B0 = K * LOGE;
+        const int KB = (LOGN - B0) < LOGE ? (LOGN - B0) : LOGE;
+        const int PER = (1 << KB) - 1;
+#pragma unroll
+        for (int gs = 0; gs < (E >> KB); ++gs) {
+            const int q = lane + C::G * gs;
+            const int rq = brev_low(q & ((1 << B0) - 1), B0);
+#pragma unroll
+            for (int i = 0; i < KB; ++i) {
+#pragma unroll
+                for (int jl = 0; jl < (1 << i); ++jl) {
+                    const int t = (rq << i) | brev_low(jl, i);  // t < 2^(B0 + i) <= N/2
+                    tw[K][gs * PER + (1 << i) - 1 + jl] = X[t];
+                    tz[K][gs * PER + (1 << i) - 1 + jl] = X[t + 1];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- load the row in natural order (grouping 0: lane holds E consecutive values)
+    cplx<T> v[E];
+    const int x0 = lane * E;
+    // (the global row type is T: rows are stored in the compute precision)
+    const cplx<T>* Yr = reinterpret_cast<const cplx<T>*>(A.Y) + row * A.in_stride;
+#pragma unroll
+    for (int u = 0; u < E; ++u) v[u] = valid ? Yr[x0 + u] : mk((T)0, (T)0);
+
+    for (int o = 0; o < RA.nmops; ++o) {
+        const int op = RA.mop[o];
+        if (op == MOP_L1F) {
+            pass_fwd<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
+        } else if (op == MOP_L1A) {
+            pass_adj<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
+        } else if (op == MOP_GRAM) {
+            gram_rec<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, lane);
+        } else if (op == MOP_BTNAT) {
+            // natural -> physical order of levels 1..n through LDS, then the adjoint passes down to grouping 0
+#pragma unroll
+            for (int u = 0; u < E; ++u) X[C::pad(lam(x0 + u, LOGN))] = v[u];
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < E; ++u) v[u] = X[C::pad(C::template phys<LAST>(lane, u))];
+            __syncthreads();
+            adj_down<T, LOGN, LOGE, LAST, LO>(v, X, tw, tz, lane);
+        } else if (op == MOP_BEND) {
+            fwd_up<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, lane);
+#pragma unroll
+            for (int u = 0; u < E; ++u) X[C::pad(C::template phys<LAST>(lane, u))] = v[u];
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < E; ++u) v[u] = X[C::pad(lam(x0 + u, LOGN))];
+            __syncthreads();
+        } else {  // MOP_P
+            const int slot = A.dslot ? A.dslot[k] : -1;
+            const bool dense = valid && slot >= 0;
+            const int d = (valid && !dense) ? A.deg[k] : 0;
+            if (lane == 0) sdeg[rib] = d * 2 + (dense ? 1 : 0);
+            __syncthreads();
+            int dmax = 0, anyd = 0;
+#pragma unroll
+            for (int r = 0; r < C::RPB; ++r) {
+                dmax = max(dmax, sdeg[r] >> 1);
+                anyd |= sdeg[r] & 1;
+            }
+            __syncthreads();
+            // The iterate is zi; with COMP it is the unevaluated sum zi + zl, and the recurrence carries its increment
+            // dl = z_j - z_(j-1) instead of the previous iterate: z_(j+1) = z_j + dl_j with dl_j = (w a) r_j + (w - 1)
+            // dl_(j-1) (the reference's z + w (a r) + (w - 1)(z - z_prev), cpp/oblique.cpp:93-97), each increment
+            // added to zi by TwoSum and its rounding error to zl.  dl starts at z_0 - 0 (the zero start).  The
+            // residual uses zi alone: tau zl is below the rounding error of the FP32 product M z.
+            constexpr int EL = COMP ? E : 1;
+            constexpr int EP = COMP ? 1 : E;
+            cplx<T> y[E], zi[E], zp[EP], zl[EL], dl[EL];
+            const double ak = valid ? A.a[k] : 0.0, ck = valid ? A.c[k] : 0.0;
+            const T aT = (T)ak, tau = (T)A.tau;
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                y[u] = v[u];
+                zi[u] = scale(aT, y[u]);
+                if constexpr (COMP) {
+                    zl[u] = mk(fma(aT, y[u].x, -zi[u].x), fma(aT, y[u].y, -zi[u].y));
+                    dl[u] = zi[u];
+                } else {
+                    zp[u] = mk((T)0, (T)0);
+                }
+            }
+            double om = 1.0;
+            for (int it = 1; it < dmax; ++it) {
+                const double omj = (it == 1) ? 1.0 / (1.0 - ck * ck / 2.0) : 1.0 / (1.0 - ck * ck * om / 4.0);
+#pragma unroll
+                for (int u = 0; u < E; ++u) v[u] = zi[u];
+                gram_rec<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, lane);
+                if (it < d) {
+                    // z <- z + (w a) r + (w - 1)(z - z_prev), r = y - M z - tau z
+                    const T wa = (T)(omj * ak), w0 = (T)(omj - 1.0);
+#pragma unroll
+                    for (int u = 0; u < E; ++u) {
+                        if constexpr (COMP) {
+                            const cplx<T> r = sub(sub(y[u], v[u]), scale(tau, zi[u]));
+                            dl[u] = add(scale(wa, r), scale(w0, dl[u]));
+                            cplx<T> hi, err;
+                            two_sum(zi[u], dl[u], hi, err);
+                            zi[u] = hi;
+                            zl[u] = add(zl[u], err);
+                        } else {
+                            const cplx<T> r = sub(sub(y[u], v[u]), scale(tau, zi[u]));
+                            const cplx<T> zn = add(add(zi[u], scale(wa, r)), scale(w0, sub(zi[u], zp[u])));
+                            zp[u] = zi[u];
+                            zi[u] = zn;
+                        }
+                    }
+                    om = omj;
+                }
+            }
+            if (anyd) {
+                // exact Tikhonov filter z = F y in double for dense rows (F^T stored: coalesced over the lane's E)
+#pragma unroll
+                for (int u = 0; u < E; ++u) X[C::pad(x0 + u)] = y[u];
+                __syncthreads();
+                if (dense) {
+                    const c128* F = A.dFT + (long long)slot * C::N * C::N;
+                    c128 acc[E];
+#pragma unroll
+                    for (int u = 0; u < E; ++u) acc[u] = mk(0.0, 0.0);
+                    for (int q = 0; q < C::N; ++q) {
+                        const c128 yq = cast<double>(X[C::pad(q)]);
+                        const c128* Fq = F + (long long)q * C::N + x0;
+#pragma unroll
+                        for (int u = 0; u < E; ++u) acc[u] = add(acc[u], mul(Fq[u], yq));
+                    }
+#pragma unroll
+                    for (int u = 0; u < E; ++u) {
+                        zi[u] = cast<T>(acc[u]);
+                        if constexpr (COMP) zl[u] = cast<T>(sub(acc[u], cast<double>(zi[u])));
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                if constexpr (COMP)
+                    v[u] = add(zi[u], zl[u]);
+                else
+                    v[u] = zi[u];
+            }
+        }
+    }
+    if (valid) {
+        cplx<T>* Zr = reinterpret_cast<cplx<T>*>(A.Z) + row * A.out_stride;
+#pragma unroll
+        for (int u = 0; u < E; ++u) Zr[x0 + u] = v[u];
+    }
+}
+
+template <typename T, int LOGN, int LOGE, bool COMP, int LO>
+int launch_cfg(const RegArgs& RA, hipStream_t s) {
+    using C = Cfg<T, LOGN, LOGE>;
+    long long nblk;
+    if (RA.group_by_k)
+        nblk = (long long)RA.nk * ((RA.SP + C::RPB - 1) / C::RPB);
+    else
+        nblk = (RA.r.rows + C::RPB - 1) / C::RPB;
+    if (nblk < 1) return OK;
+    if (nblk > 0x7fffffffLL) {
+        set_error("rows_reg: too many blocks");
+        return E_INVALID;
+    }
+    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RA);
+    return launch_check("rows_reg_kernel");
+}
+
+template <typename T, int LOGN, bool COMP, int LO>
+int launch_n(const RegArgs& RA, int loge, hipStream_t s) {
+    if constexpr (LOGN >= 2) {
+        if (loge == 2) return launch_cfg<T, LOGN, 2, COMP, LO>(RA, s);
+    }
+    if constexpr (LOGN >= 3) {
+        if (loge == 3) return launch_cfg<T, LOGN, 3, COMP, LO>(RA, s);
+    }
+    if constexpr (LOGN >= 4) {
+        if (loge == 4) return launch_cfg<T, LOGN, 4, COMP, LO>(RA, s);
+    }
+    set_error("rows_reg: unsupported elements-per-lane");
+    return E_INVALID;
+}
+
+template <typename T, bool COMP, int LO>
+int launch_t(const RegArgs& RA, int loge, hipStream_t s) {
+    switch (RA.r.n) {
+        case 5: return launch_n<T, 5, COMP, LO>(RA, loge, s);
+        case 6: return launch_n<T, 6, COMP, LO>(RA, loge, s);
+        case 7: return launch_n<T, 7, COMP, LO>(RA, loge, s);
+        case 8: return launch_n<T, 8, COMP, LO>(RA, loge, s);
+        case 9: return launch_n<T, 9, COMP, LO>(RA, loge, s);
+        case 10: return launch_n<T, 10, COMP, LO>(RA, loge, s);
+        case 11: return launch_n<T, 11, COMP, LO>(RA, loge, s);
+        default: set_error("rows_reg: n must be 5..11"); return E_INVALID;
+    }
+}
+
+}  // namespace rreg
+
+}  // namespace adrt

@@ -1,0 +1,8 @@
+// Register row kernel instantiations: compute type float, compensated=true, Chebyshev / Gram levels 1..n.
+#include "rows_reg.hpp"
+
+namespace adrt {
+
+int rows_reg_launch_c1(const RegArgs& RA, int loge, hipStream_t s) { return rreg::launch_t<float, true, 1>(RA, loge, s); }
+
+}  // namespace adrt

@@ -181,6 +181,7 @@ def rows_only(a):
     H64 = I.FastHybridHip(n, dense_cap=cap, precision="fp64", max_batch=1)
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B,
                           outer=(H64.lo, H64.hi, H64.outer_deg), refine=a.refine)
+    pkg("autotune").tune(H32, B, persist=False)
     st = L.Stream()
     d_x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal((B, N, N)), st)
     d_y = L.DeviceArray((B, N, N))
@@ -238,6 +239,9 @@ def main():
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
                           refine=a.refine)
     log(f"setup {setup_s:.1f}s: D={H64.outer_deg} sum(deg)/N={H64.deg.sum() / N:.1f} dense rows={H64.eng.n_dense}")
+    # the row kernel's mapping for this (N, FP32) chosen by the project's autotuner on the plan's own row pass
+    loge, tune_ms = pkg("autotune").tune(H32, B, persist=False)
+    log(f"autotuned row mapping: loge={loge} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tune_ms.items())})")
 
     # inputs: ellipse phantoms (seeds 1000+i) through the GPU forward ADRT, plus 1% noise (seeds 2000+i)
     Bn = make_sinos(n, B, 1000, 2000)
@@ -340,7 +344,8 @@ def main():
         "data": "synthetic: ellipse phantoms (seeds 1000+i) + 1% Gaussian noise (seeds 2000+i)",
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
                    "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
-                   "sum_deg_over_N": float(H64.deg.sum() / N), "n_dense": H64.eng.n_dense},
+                   "sum_deg_over_N": float(H64.deg.sum() / N), "n_dense": H64.eng.n_dense,
+                   "row_mapping_loge": int(loge)},
         "plain_fp64_inverses_per_s": plain_ips,
         "speedup_vs_plain_fp64": ips / plain_ips,
         "relerr_fp32_vs_fp64_max": float(rel.max()),
