@@ -236,11 +236,29 @@ int adrt_free(void* ptr) {
 int adrt_memcpy(void* dst, const void* src, unsigned long long bytes, int kind, void* stream) {
     if (bytes == 0) return OK;
     ADRT_REQUIRE(dst && src, "adrt_memcpy: null pointer");
-    ADRT_REQUIRE(kind >= 0 && kind <= 3, "adrt_memcpy: kind must be 0..3 (h2h, h2d, d2h, d2d)");
-    const hipMemcpyKind k[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
-                                hipMemcpyDeviceToDevice};
+    ADRT_REQUIRE(kind >= 0 && kind <= 4, "adrt_memcpy: kind must be 0..4 (h2h, h2d, d2h, d2d, d2h async)");
+    const hipMemcpyKind k[5] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                                hipMemcpyDeviceToDevice, hipMemcpyDeviceToHost};
     ADRT_HIP(hipMemcpyAsync(dst, src, bytes, k[kind], (hipStream_t)stream));
     if (kind == 2 || kind == 0) ADRT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return OK;
+}
+
+int adrt_host_malloc(void** ptr, unsigned long long bytes) {
+    ADRT_REQUIRE(ptr, "adrt_host_malloc: null pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return OK;
+    const hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        return E_NOMEM;
+    }
+    return OK;
+}
+
+int adrt_host_free(void* ptr) {
+    if (ptr) ADRT_HIP(hipHostFree(ptr));
     return OK;
 }
 

@@ -34,6 +34,8 @@ SIGNATURES = {
     "adrt_free": (I, [P]),
     "adrt_memcpy": (I, [P, P, ULL, I, P]),
     "adrt_memset": (I, [P, I, ULL, P]),
+    "adrt_host_malloc": (I, [ctypes.POINTER(P), ULL]),
+    "adrt_host_free": (I, [P]),
     "adrt_stream_create": (I, [ctypes.POINTER(P)]),
     "adrt_stream_destroy": (I, [P]),
     "adrt_stream_sync": (I, [P]),
@@ -167,6 +169,33 @@ class Stream:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """Page-locked host buffer (adrt_host_malloc) seen as a numpy array ``a``: host<->device copies from or into it
+    run asynchronously on their stream."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        h = P()
+        check(load().adrt_host_malloc(ctypes.byref(h), max(1, self.nbytes)), "host_malloc")
+        self.ptr = h.value
+        buf = (ctypes.c_char * max(1, self.nbytes)).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape, dtype=np.int64))).reshape(self.shape)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.a = None
+            load().adrt_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

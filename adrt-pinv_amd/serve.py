@@ -6,7 +6,8 @@ stream, input / output buffers for up to ``max_batch`` sinograms and a completio
 
   * polls the slots' events; a finished slot hands its results back and becomes free again,
   * admits waiting requests of a free slot's (N, p) as one batch (up to max_batch) and launches the whole inverse on
-    the slot's stream,
+    the slot's stream (upload from page-locked memory, one hipGraph replay of the inverse for that batch size,
+    download into page-locked memory, completion event),
 
 so requests that arrive while others are on the GPU start as soon as a slot of their size frees up, and the batches of
 different sizes overlap on the device.  Slots are created at start-up while their device memory fits the budget;
@@ -32,7 +33,9 @@ class Request:
 
 
 class GpuSlot:
-    """One resident device plan plus buffers, stream and event."""
+    """One resident device plan plus buffers, stream and event.  The sinograms go up from and the images come back into
+    page-locked host buffers on the slot's stream; the inverse itself is replayed from a hipGraph captured once per
+    batch size, so a launch costs three enqueues and the completion is one event."""
 
     def __init__(self, n, cap, precision="fp32", refine=2, max_batch=4, setup=None, **kw):
         from . import lib as L
@@ -48,7 +51,9 @@ class GpuSlot:
         self.d_b = L.DeviceArray((max_batch, 4, 2 * N - 1, N))
         self.d_x = L.DeviceArray((max_batch, N, N))
         self.event = L.Event()
-        self.host_b = np.zeros((max_batch, 4, 2 * N - 1, N))
+        self.host_b = L.PinnedArray((max_batch, 4, 2 * N - 1, N))
+        self.host_x = L.PinnedArray((max_batch, N, N))
+        self.graphs = {}
         self.busy = []
 
     @staticmethod
@@ -63,37 +68,53 @@ class GpuSlot:
     def shared_bytes(self):
         return getattr(self.plan, "table_bytes", 0)
 
+    def prepare(self):
+        """Capture the graphs of every batch size up front (the first request of a size then pays no capture)."""
+        for S in range(1, self.max_batch + 1):
+            self.graph(S)
+
+    def graph(self, S):
+        g = self.graphs.get(S)
+        if g is None:
+            with self.L.Graph(self.stream) as g:
+                self.plan.apply_device(self.d_b, self.d_x, S, self.stream, count=False)
+            self.graphs[S] = g
+        return g
+
     def launch(self, reqs):
         S = len(reqs)
         for j, r in enumerate(reqs):
-            self.host_b[j] = r.b
-        L = self.L
-        L.check(L.load().adrt_memcpy(self.d_b.ptr, self.host_b.ctypes.data, S * self.host_b[0].nbytes, 1,
-                                     self.stream.handle), "h2d")
-        self.plan.apply_device(self.d_b, self.d_x, S, self.stream, count=False)
+            self.host_b.a[j] = r.b
+        L, lib = self.L, self.L.load()
+        g = self.graph(S)
+        L.check(lib.adrt_memcpy(self.d_b.ptr, self.host_b.ptr, S * self.host_b.a[0].nbytes, 1, self.stream.handle),
+                "h2d")
+        g.launch(self.stream)
+        L.check(lib.adrt_memcpy(self.host_x.ptr, self.d_x.ptr, S * self.host_x.a[0].nbytes, 4, self.stream.handle),
+                "d2h")
         self.event.record(self.stream)
         self.busy = reqs
 
     def poll(self):
         if not self.busy or not self.event.done():
             return []
-        S = len(self.busy)
-        x = np.empty((S, 1 << self.n, 1 << self.n))
-        L = self.L
-        L.check(L.load().adrt_memcpy(x.ctypes.data, self.d_x.ptr, x.nbytes, 2, self.stream.handle), "d2h")
         done, self.busy = self.busy, []
         for j, r in enumerate(done):
-            r.x = x[j]
+            r.x = self.host_x.a[j].copy()
         return done
 
     def close(self):
+        self.graphs.clear()
         self.plan.close()
+        self.host_b.free()
+        self.host_x.free()
 
 
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
                  precision="fp32", refine=2, **kw):
-        """keys: iterable of (n, cap).  mem_budget: device bytes the slots may hold (default: 60% of free memory)."""
+        """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
+        deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory)."""
         self.keys = list(keys)
         self.pending = {k: deque() for k in self.keys}
         self.slots = []
@@ -111,7 +132,7 @@ class Server:
         used = 0
         for rnd in range(slots_per_key):
             for n, cap in self.keys:
-                s = factory(n, cap, max_batch)
+                s = factory(n, cap, max_batch.get(n, 4) if isinstance(max_batch, dict) else max_batch)
                 b = s.device_bytes() + (getattr(s, "shared_bytes", lambda: 0)() if rnd == 0 else 0)
                 if mem_budget is not None and used + b > mem_budget and rnd > 0:
                     s.close()  # admission by memory: no further instances of this key
@@ -120,6 +141,9 @@ class Server:
                 self.slots.append(s)
         self.device_bytes = used
         self.served = {k: any((s.n, s.cap) == k for s in self.slots) for k in self.keys}
+        for s in self.slots:
+            if hasattr(s, "prepare"):
+                s.prepare()
 
     def submit(self, req):
         k = (req.n, req.cap)

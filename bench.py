@@ -146,29 +146,36 @@ def mixed_batch_leg(count=512, dense_cap=16):
             "setup_s": setup_s, "row_mapping_loge": tuned, "dense_cap": dense_cap}
 
 
-def serving_leg(rate, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), max_batch=4,
-                slots_per_key=2):
-    """Poisson arrivals (default_rng(7)) of FP32 hybrid requests over mixed N and p; p50 / p99 latency."""
+SERVE_BATCH = {7: 16, 8: 8, 9: 4, 10: 2}  # sinograms per launch by log2 N (small N batch deeper)
+
+
+def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), slots_per_key=2):
+    """Poisson arrivals (default_rng(7)) of FP32 hybrid requests over mixed N and p at each rate; p50 / p99 latency
+    and throughput (the request's upload and download are inside its latency)."""
     S = pkg("serve")
     C = pkg("plancache")
     keys = [(n, c) for n in sizes for c in caps]
     t0 = time.time()
     setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
+    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=SERVE_BATCH, setups=setups)
     setup_s = time.time() - t0
-    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=max_batch, setups=setups)
     pool = {n: list(make_sinos(n, 4, 3000, 4000)) for n in sizes}
     # warm every slot once (first launches load code objects)
     warm = [(0.0, S.Request(-1 - i, s.n, s.cap, pool[s.n][0], 0.0)) for i, s in enumerate(srv.slots)]
     srv.run_trace(warm)
-    srv.done.clear()
-    trace = S.poisson_trace(rate, duration, list(sizes), list(weights), list(caps), pool)
-    done = srv.run_trace(trace)
-    st = S.latency_stats(done)
-    out = {"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **st,
-           "mix": "N in {128,256,512,1024} w .4/.3/.2/.1, p in {16,32}", "slots": len(srv.slots),
-           "max_batch": max_batch, "device_GiB": srv.device_bytes / 2**30, "setup_s": setup_s}
+    out = []
+    for rate in rates:
+        srv.done.clear()
+        trace = S.poisson_trace(rate, duration, list(sizes), list(weights), list(caps), pool)
+        done = srv.run_trace(trace)
+        out.append({"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **S.latency_stats(done)})
+        log(f"serving at {rate:.0f}/s: p50 {out[-1].get('p50_ms', 0):.1f} ms p99 {out[-1].get('p99_ms', 0):.1f} ms "
+            f"throughput {out[-1].get('throughput_per_s', 0):.0f}/s")
+    res = {"mix": "N in {128,256,512,1024} w .4/.3/.2/.1, p in {16,32}", "slots": len(srv.slots),
+           "max_batch": {str(1 << n): b for n, b in SERVE_BATCH.items()}, "device_GiB": srv.device_bytes / 2**30,
+           "setup_s": setup_s, "rates": out}
     srv.close()
-    return out
+    return res
 
 
 def rows_only(a):
@@ -213,7 +220,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-rates", default="200,400", help="Poisson arrival rates (req/s), comma separated")
     ap.add_argument("--serve-duration", type=float, default=2.0)
     ap.add_argument("--rows-only", action="store_true")
     a = ap.parse_args()
@@ -312,9 +319,7 @@ def main():
         log(f"mixed batch: {mixed['inverses_per_s']:.0f} inverses/s")
     serving = None
     if not a.no_serve:
-        serving = serving_leg(a.serve_rate, a.serve_duration)
-        log(f"serving at {a.serve_rate:.0f}/s: p50 {serving.get('p50_ms', 0):.1f} ms p99 "
-            f"{serving.get('p99_ms', 0):.1f} ms")
+        serving = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_duration)
 
     cpu = None
     if not a.no_cpu:
@@ -328,6 +333,12 @@ def main():
                              "NumPy host (FFTs, packing, outer loop)",
                    "relerr_vs_gpu_fp64": float(np.linalg.norm(xr - x64[0]) / np.linalg.norm(x64[0]))}
             log(f"cpu baseline {tmin:.2f} s per inverse")
+            # one thread, as the paper times the engine (README.md:65-66): a single run
+            res1 = cpu_baseline(H64, cap, Bn[0], 1, budget_s=0.0)
+            if res1 is not None:
+                cpu["one_thread"] = {"value": 1.0 / res1[0], "unit": "inverses/s", "cores": 1,
+                                     "sample": f"1 hybrid inverse at N={N}, one run"}
+                log(f"cpu baseline, one thread: {res1[0]:.2f} s per inverse")
 
     line = {
         "metric": "hybrid inverses/s at N=256, FP32 vs FP64 (relerr<=1e-5); p99 latency mixed-N",

@@ -56,8 +56,12 @@ const char* adrt_last_error(void); /* message of the last failure on this thread
 int adrt_device_count(void);
 int adrt_malloc(void** ptr, unsigned long long bytes);
 int adrt_free(void* ptr);
-/* kind: 0 host->host, 1 host->device, 2 device->host (synchronous), 3 device->device (stream-ordered) */
+/* kind: 0 host->host, 1 host->device, 2 device->host (synchronous), 3 device->device (stream-ordered),
+ * 4 device->host stream-ordered (returns at once; for page-locked destinations from adrt_host_malloc) */
 int adrt_memcpy(void* dst, const void* src, unsigned long long bytes, int kind, void* stream);
+/* page-locked host memory: host<->device copies from / to it run asynchronously on the stream */
+int adrt_host_malloc(void** ptr, unsigned long long bytes);
+int adrt_host_free(void* ptr);
 int adrt_memset(void* dst, int value, unsigned long long bytes, void* stream);
 int adrt_stream_create(void** stream);
 int adrt_stream_destroy(void* stream);
