@@ -124,7 +124,7 @@ int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops
 int precond(Plan* p, const double* v, double* out, int S, hipStream_t st) {
     const int n = p->d.n;
     const long long count = (long long)S * p->N * p->N;
-    if (p->d.kind == ADRT_KIND_LP || p->d.precond == ADRT_PRECOND_NONE) {
+    if (p->d.kind != ADRT_KIND_HYBRID || p->d.precond == ADRT_PRECOND_NONE) {
         if (out != v) ADRT_HIP(hipMemcpyAsync(out, v, sizeof(double) * count, hipMemcpyDeviceToDevice, st));
         return OK;
     }
@@ -179,7 +179,8 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
 int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     ADRT_REQUIRE(out && d, "plan_create: null pointer");
     *out = nullptr;
-    ADRT_REQUIRE(d->kind == ADRT_KIND_LP || d->kind == ADRT_KIND_HYBRID, "plan_create: kind must be LP or HYBRID");
+    ADRT_REQUIRE(d->kind == ADRT_KIND_LP || d->kind == ADRT_KIND_HYBRID || d->kind == ADRT_KIND_NORMAL,
+                 "plan_create: kind must be LP, HYBRID or NORMAL");
     ADRT_REQUIRE(d->n >= 1 && d->n <= 11, "plan_create: need 1 <= n <= 11");
     ADRT_REQUIRE(d->precision >= ADRT_PREC_F64 && d->precision <= ADRT_PREC_F32_PLAIN,
                  "plan_create: precision must be 0, 1 or 2");
@@ -190,7 +191,9 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     ADRT_REQUIRE(d->max_batch >= 1, "plan_create: max_batch must be >= 1");
     ADRT_REQUIRE(d->refine >= 0, "plan_create: refine must be >= 0");
     ADRT_REQUIRE(d->dense_on_device >= 0 && d->dense_on_device <= 2, "plan_create: dense_on_device must be 0, 1 or 2");
-    ADRT_REQUIRE(d->z && d->deg && d->a && d->c, "plan_create: null row parameter array");
+    ADRT_REQUIRE(d->z && ((d->deg && d->a && d->c) || d->kind == ADRT_KIND_NORMAL),
+                 "plan_create: null row parameter array");
+    ADRT_REQUIRE(d->kind != ADRT_KIND_NORMAL || d->n_dense == 0, "plan_create: the normal operator has no dense rows");
     ADRT_REQUIRE(d->n_dense >= 0 && d->n_dense <= (2 << d->n), "plan_create: bad n_dense");
     ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && d->dense_F), "plan_create: dense rows need dense_k and dense_F");
     auto* p = new Plan();
@@ -221,9 +224,15 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     };
     int st;
     if ((st = alloc((void**)&p->z, sizeof(c128) * L)) || (st = up(p->z, d->z, sizeof(c128) * L))) return fail(st);
-    if ((st = alloc((void**)&p->deg, sizeof(int) * L)) || (st = up(p->deg, d->deg, sizeof(int) * L))) return fail(st);
-    if ((st = alloc((void**)&p->a, sizeof(double) * L)) || (st = up(p->a, d->a, sizeof(double) * L))) return fail(st);
-    if ((st = alloc((void**)&p->c, sizeof(double) * L)) || (st = up(p->c, d->c, sizeof(double) * L))) return fail(st);
+    // (the normal operator has no Chebyshev rows: its deg / a / c may be null and are zero)
+    const std::vector<int> zdeg(d->deg ? 0 : L, 0);
+    const std::vector<double> zac(d->a && d->c ? 0 : L, 0.0);
+    const int* degs = d->deg ? d->deg : zdeg.data();
+    const double* as = d->a && d->c ? d->a : zac.data();
+    const double* cs = d->a && d->c ? d->c : zac.data();
+    if ((st = alloc((void**)&p->deg, sizeof(int) * L)) || (st = up(p->deg, degs, sizeof(int) * L))) return fail(st);
+    if ((st = alloc((void**)&p->a, sizeof(double) * L)) || (st = up(p->a, as, sizeof(double) * L))) return fail(st);
+    if ((st = alloc((void**)&p->c, sizeof(double) * L)) || (st = up(p->c, cs, sizeof(double) * L))) return fail(st);
     if (d->n_dense > 0) {
         std::vector<int> slot(L, -1);
         for (int i = 0; i < d->n_dense; ++i) {
@@ -295,7 +304,7 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
         std::vector<long long> key(L);
         for (int k = 0; k < L; ++k) {
             ord[k] = k;
-            key[k] = (long long)d->deg[k];
+            key[k] = (long long)degs[k];
         }
         for (int i = 0; i < d->n_dense; ++i) key[d->dense_k[i]] = 1LL << 40;
         std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[x] > key[y]; });
@@ -345,6 +354,13 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
         p->nop = 5;
         for (int i = 0; i < 3; ++i) p->ops_back[i] = back[i];
         for (int i = 0; i < 5; ++i) p->ops_op[i] = gh[i];
+    } else if (d->kind == ADRT_KIND_NORMAL) {
+        // R^T and R^T R (patches/adrt-gram-freq.patch:454-466)
+        p->nback = 1;
+        p->nop = 2;
+        p->ops_back[0] = RowOp{OP_BT, 1, n};
+        p->ops_op[0] = RowOp{OP_B, 1, n};
+        p->ops_op[1] = RowOp{OP_BT, 1, n};
     } else {
         const RowOp back[2] = {{OP_BT, 1, n}, {OP_P, 1, n}};
         const RowOp fr[3] = {{OP_P, 1, n}, {OP_B, 1, n}, {OP_BT, 1, n}};
@@ -436,7 +452,7 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     double* X[2];
     X[(D - 1) % 2] = d_x;
     X[D % 2] = p->xs;
-    const bool has_p = !(p->d.kind == ADRT_KIND_LP || p->d.precond == ADRT_PRECOND_NONE);
+    const bool has_p = p->d.kind == ADRT_KIND_HYBRID && p->d.precond != ADRT_PRECOND_NONE;
     // g1_inv fused with the outer scale / update when its degree fits the fused kernel's halo
     const bool fused = has_p && p->d.precond == ADRT_PRECOND_G1 && p->d.g1_deg - 1 <= 16;
     const double* py = p->y;

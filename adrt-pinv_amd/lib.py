@@ -87,7 +87,7 @@ class PlanDesc(ctypes.Structure):
     ]
 
 
-KIND_LP, KIND_HYBRID = 0, 1
+KIND_LP, KIND_HYBRID, KIND_NORMAL = 0, 1, 2
 PRECOND = {"none": 0, "g1": 1, "lorr": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_plain": 2}
 # the row-kernel configuration table is kept per compute type: both FP32 variants share one entry

@@ -96,7 +96,12 @@ int adrt_rows_config_get(int n, int precision);
 /* Inverse plans: fastops.FastLp / FastHybrid (fastops.py:261-338) with the setup scalars given.
  * All image / sinogram buffers are float64 on the device:
  *   b: (S, 4, 2N-1, N) sinograms, x / y: (S, N, N) images. */
-enum { ADRT_KIND_LP = 0, ADRT_KIND_HYBRID = 1 };
+/* KIND_NORMAL: the normal operator R^T R of the ADRT row by row in frequency (the row Gram gram_freq, ops B 1..n then
+ * BT 1..n on the packed quadrant pairs, patches/adrt-gram-freq.patch:220-251, 411-471) with backprojection R^T
+ * (ops BT 1..n) and no filter or preconditioner: its apply is the fixed-degree Chebyshev semi-iteration on the normal
+ * equations on [lo, hi] = [lam_min, lam_max] with outer_deg steps (lstsq_cheb, patch:474-545).  deg, a and c may be
+ * null; n_dense must be 0. */
+enum { ADRT_KIND_LP = 0, ADRT_KIND_HYBRID = 1, ADRT_KIND_NORMAL = 2 };
 enum { ADRT_PRECOND_NONE = 0, ADRT_PRECOND_G1 = 1, ADRT_PRECOND_LORR = 2 };
 enum { ADRT_PREC_F64 = 0, ADRT_PREC_F32 = 1, ADRT_PREC_F32_PLAIN = 2 };
 typedef struct {

@@ -1,4 +1,5 @@
-"""Least-squares cross-check by CGLS, restated (oracle / test infrastructure only).  Follows pinv.py:23-48."""
+"""Least-squares comparators, restated (oracle / test infrastructure only): CGLS (pinv.py:23-48) and the normal-equation
+Chebyshev lstsq_cheb with its Gram (patches/adrt-gram-freq.patch:411-545)."""
 
 import numpy as np
 
@@ -40,3 +41,55 @@ def pinv_dense(n):
     N = 1 << n
     A = radon.dense(lambda x: radon.adrt(n, x), (N, N))
     return np.linalg.pinv(A)
+
+
+def gram(n, a):
+    """R^T R a through the spatial ADRT pair: the reference against which the patch tests its frequency-row Gram
+    (patches/adrt-gram-freq.patch:592-594: truncate(bdrt(adrt(a))).sum(axis=-3)), in this repository's layout."""
+    N = 1 << n
+    a = np.asarray(a, dtype=float).reshape(N, N)
+    return radon.adrt_T(n, radon.adrt(n, a))
+
+
+def cheb_degree(lo, hi, tol):
+    """Degree of lstsq_cheb for the interval [lo, hi] and tolerance tol (patch:525-532)."""
+    kappa = hi / lo
+    rho = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
+    return max(1, int(np.ceil(np.log(tol / 2) / np.log(rho))))
+
+
+def power_lam_max(n, op=None, iters=50):
+    """Upper bound of lambda_max(R^T R) as lstsq_cheb estimates it (patch:512-520): 50 normalised power steps from
+    default_rng(0).standard_normal((N, N)), then theta = <v, Gv> enlarged by ||Gv - theta v||."""
+    N = 1 << n
+    op = op or (lambda v: gram(n, v))
+    v = np.random.default_rng(0).standard_normal((N, N))
+    v /= np.linalg.norm(v)
+    for _ in range(iters):
+        w = op(v)
+        v = w / np.linalg.norm(w)
+    w = op(v)
+    theta = float(np.vdot(v, w))
+    return theta + float(np.linalg.norm(w - theta * v))
+
+
+def lstsq_cheb(n, b, lam_min, lam_max=None, tol=1e-10, degree=None, op=None):
+    """x_D = c_D(R^T R) R^T b, the fixed-degree Chebyshev semi-iteration on the normal equations over
+    [lam_min, lam_max] (patches/adrt-gram-freq.patch:474-545), with y = R^T b = adrt_T(b) in this layout."""
+    N = 1 << n
+    op = op or (lambda v: gram(n, v))
+    y = radon.adrt_T(n, np.asarray(b, dtype=float).reshape(4, 2 * N - 1, N))
+    if lam_max is None:
+        lam_max = power_lam_max(n, op)
+    lo, hi = float(lam_min), float(lam_max)
+    if not 0 < lo < hi:
+        raise ValueError("need 0 < lam_min < lam_max")
+    deg = cheb_degree(lo, hi, tol) if degree is None else int(degree)
+    c, a = (hi - lo) / (hi + lo), 2.0 / (hi + lo)
+    x_prev, x = np.zeros_like(y), a * y
+    om = 1.0
+    for k in range(1, deg):
+        om = 1.0 / (1.0 - c * c / 2.0) if k == 1 else 1.0 / (1.0 - c * c * om / 4.0)
+        r = y - op(x)
+        x_prev, x = x, x + om * (a * r) + (om - 1.0) * (x - x_prev)
+    return x

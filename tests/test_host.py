@@ -142,3 +142,17 @@ def test_poisson_trace_and_stats():
         r.t_arrive, r.t_done = 0.0, 0.001 * (i + 1)
     st = S.latency_stats([r for _, r in tr])
     assert st["count"] == len(tr) and st["p99_ms"] >= st["p50_ms"]
+
+
+def test_gram_and_lstsq_cheb_validation():
+    """The argument checks of adrt.core.gram / lstsq_cheb (patch:629-639, 712-714) run before any device call."""
+    lsq = pkg("lsq")
+    for shape in [(4, 8), (6, 6), (8,)]:
+        with pytest.raises(ValueError):
+            lsq.gram(np.zeros(shape))
+    with pytest.raises(TypeError):
+        lsq.gram(np.zeros((8, 8), dtype=np.int64))
+    with pytest.raises(ValueError):
+        lsq.lstsq_cheb(np.zeros((4, 15, 8)), lam_min=10.0, lam_max=5.0)
+    with pytest.raises(ValueError):
+        lsq.lstsq_cheb(np.zeros((4, 8, 8)), lam_min=1.0, lam_max=5.0)

@@ -280,3 +280,50 @@ def test_reference_cheb_and_cert():
     assert relmax(Z1, Z2) < 1e-13 and s1 == s2
     om = 2 * np.pi * np.fft.fftfreq(L)
     assert np.array_equal(refcpp.cert_rowsum(n, om, 2), oiv.cert_bound(n, om, 2))
+
+
+# ---- normal-equation Chebyshev (patches/adrt-gram-freq.patch:474-545) and its Gram
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_lstsq_cheb_matches_dense_lstsq(n):
+    """The patch's own check (tests/test_lstsq_cheb.py, patch:694-701): against numpy's least squares on the dense ADRT
+    with an interval 1% outside the extreme eigenvalues of R^T R."""
+    N = 1 << n
+    rng = np.random.default_rng(N)
+    A = radon.dense(lambda x: radon.adrt(n, x), (N, N))
+    eig = np.linalg.eigvalsh(A.T @ A)
+    b = radon.adrt(n, rng.standard_normal((N, N))) + 0.1 * rng.standard_normal((4, 2 * N - 1, N))
+    ref = np.linalg.lstsq(A, b.ravel(), rcond=None)[0].reshape(N, N)
+    out = lsq.lstsq_cheb(n, b, lam_min=0.99 * eig[0], lam_max=1.01 * eig[-1], tol=1e-10)
+    assert np.linalg.norm(out - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def test_lstsq_cheb_estimated_lam_max():
+    """patch:704-709: lam_min = 15 (below lambda_min = 15.47 at N = 8) and the power estimate recover R^-1 R x = x."""
+    n = 3
+    x = np.random.default_rng(3).standard_normal((8, 8))
+    out = lsq.lstsq_cheb(n, radon.adrt(n, x), lam_min=15.0)
+    assert np.linalg.norm(out - x) <= 1e-8 * np.linalg.norm(x)
+    with pytest.raises(ValueError):
+        lsq.lstsq_cheb(n, radon.adrt(n, np.zeros((8, 8))), lam_min=10.0, lam_max=5.0)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3])
+def test_gram_dense_and_power_bound(n):
+    """R^T R a equals the dense A^T A a; it is symmetric positive; the power estimate bounds lambda_max from above to
+    within its residual; the degree is the smallest D with 2 rho^D <= tol."""
+    N = 1 << n
+    A = radon.dense(lambda x: radon.adrt(n, x), (N, N))
+    rng = np.random.default_rng(n)
+    u, v = rng.standard_normal((2, N, N))
+    assert np.allclose(lsq.gram(n, u).ravel(), A.T @ (A @ u.ravel()), rtol=0, atol=1e-12 * np.abs(A).sum() ** 2)
+    assert np.isclose(np.vdot(lsq.gram(n, u), v), np.vdot(u, lsq.gram(n, v)))
+    assert np.vdot(lsq.gram(n, u), u) > 0
+    lam = np.linalg.eigvalsh(A.T @ A)[-1]
+    est = lsq.power_lam_max(n)
+    assert lam <= est * (1 + 1e-12) and est <= lam * (1 + 1e-3)
+    lo, hi, tol = 1.0, 50.0, 1e-10
+    D = lsq.cheb_degree(lo, hi, tol)
+    rho = (np.sqrt(hi / lo) - 1) / (np.sqrt(hi / lo) + 1)
+    assert 2 * rho**D <= tol < 2 * rho ** (D - 1)
