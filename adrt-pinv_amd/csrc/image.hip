@@ -186,6 +186,25 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
         if (b < NB) X[lds[e]] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
     }
     __syncthreads();
+    // the outer update's x and x_prev of the tile's cells (the first TB^2 cells, slots e < EPI), loaded now so that
+    // their latency hides behind the inner iteration
+    constexpr int EPI = (TB * TB + G1_THREADS - 1) / G1_THREADS;
+    double xo_[EPI][4], xpo_[EPI][4];
+    if (A.mode == 2) {
+#pragma unroll
+        for (int e = 0; e < EPI; ++e) {
+            const int b = threadIdx.x + G1_THREADS * e;
+            const int bi = HB + b / TB, bj = HB + b % TB;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int gi = gi0 + 2 * bi + (c >> 1), gj = gj0 + 2 * bj + (c & 1);
+                const bool in = b < TB * TB && gi < N && gj < N;
+                const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
+                xo_[e][c] = in ? A.x[o] : 0.0;
+                xpo_[e][c] = (in && A.xp) ? A.xp[o] : 0.0;
+            }
+        }
+    }
     for (int k = 1; k <= H; ++k) {
         const T om = (T)A.om[k - 1], om1 = (T)(A.om[k - 1] - 1.0);
         const Cell4<T>* Xr = X + ((k - 1) & 1) * BUF;
@@ -237,9 +256,9 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
                 A.out[o] = pv;
             } else if (A.mode == 1) {
                 A.out[o] = A.a_out * pv;
-            } else {
-                const double xo = A.x[o];
-                const double dx = A.xp ? (xo - A.xp[o]) : (xo - 0.0);
+            } else if (e < EPI) {
+                const double xo = xo_[e][c];
+                const double dx = A.xp ? (xo - xpo_[e][c]) : (xo - 0.0);
                 A.out[o] = xo + A.w1 * (A.a_out * pv) + A.w0 * dx;
             }
         }

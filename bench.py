@@ -2,6 +2,7 @@
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 8] [--dense-cap 16] [--no-cpu] [--no-serve]
     python bench.py --rows-only      # short workload for rocprofv3 --pmc runs of the row kernel
+    python bench.py --large          # adds the N=2048, batch-16 FP32 vs FP64 leg
 
 A step is one batched hybrid inverse (FP32 row engine with TwoSum-compensated Chebyshev accumulation; FP64 dense
 table GEMM, preconditioner, outer iteration and the last refinement steps) of B noisy ellipse-phantom sinograms that
@@ -178,6 +179,44 @@ def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.
     return res
 
 
+def large_leg(n=11, batch=16, dense_cap=16):
+    """N=2048 hybrid, batch 16: ellipse phantoms (seeds 5000+i) + 0.5% noise (seeds 6000+i); the FP32 plan (TwoSum,
+    FP64 dense rows / outer / refinement) against the FP64 plan on the same batch, both one device call."""
+    L = pkg("lib")
+    I = pkg("inverses")
+    N = 1 << n
+    t0 = time.time()
+    H64 = I.FastHybridHip(n, dense_cap=dense_cap, precision="fp64", max_batch=batch)
+    H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=dense_cap, precision="fp32", max_batch=batch,
+                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=2)
+    setup_s = time.time() - t0
+    Bn = make_sinos(n, batch, 5000, 6000, level=0.005)
+    st = L.Stream()
+    d_b = L.DeviceArray.from_host(Bn, st)
+    d_x = L.DeviceArray((batch, N, N))
+    e0, e1 = L.Event(), L.Event()
+    out, x = {}, {}
+    for name, H in (("fp64", H64), ("fp32", H32)):
+        H.apply_device(d_b, d_x, batch, st, count=False)
+        st.sync()
+        e0.record(st)
+        H.apply_device(d_b, d_x, batch, st, count=False)
+        e1.record(st)
+        ms = e0.elapsed_ms(e1)
+        x[name] = d_x.download(st)
+        out[f"{name}_inverses_per_s"] = batch * 1000.0 / ms
+    rel = [float(np.linalg.norm(x["fp32"][i] - x["fp64"][i]) / np.linalg.norm(x["fp64"][i])) for i in range(batch)]
+    res = {"workload": f"hybrid inverse N={N}, batch {batch}, dense_cap={dense_cap}, 0.5% noise", **out,
+           "relerr_fp32_vs_fp64_max": max(rel), "relerr_fp32_vs_fp64_median": float(np.median(rel)),
+           "outer_deg": H64.outer_deg, "n_dense": H64.eng.n_dense, "table_GiB": H64.table_bytes / 2**30,
+           "setup_s": setup_s}
+    H32.close()
+    H64.close()
+    log(f"large N={N}: fp32 {out['fp32_inverses_per_s']:.2f}/s, fp64 {out['fp64_inverses_per_s']:.2f}/s, "
+        f"relerr max {max(rel):.2e}")
+    return res
+
+
 def rows_only(a):
     """Short workload for hardware-counter runs: K launches of the row kernel of the G_h pass at the headline
     configuration, then one device-to-device copy of a known size (the counter calibration)."""
@@ -223,6 +262,8 @@ def main():
     ap.add_argument("--serve-rates", default="200,400", help="Poisson arrival rates (req/s), comma separated")
     ap.add_argument("--serve-duration", type=float, default=2.0)
     ap.add_argument("--rows-only", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="also the N=2048 batch-16 FP32 vs FP64 leg (about 45 s of setup and 50 GiB of dense tables)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the sinogram batch would shard by "
@@ -321,6 +362,8 @@ def main():
     if not a.no_serve:
         serving = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_duration)
 
+    large = large_leg() if a.large else None
+
     cpu = None
     if not a.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -368,6 +411,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_batch": mixed,
         "serving": serving,
+        "large": large,
     }
     if cpu:
         line["speedup_vs_cpu"] = ips / cpu["value"]
