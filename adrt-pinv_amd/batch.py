@@ -16,7 +16,7 @@ from .inverses import FastHybridHip, FastLpHip
 
 
 class MixedBatch:
-    def __init__(self, ns, kind="hybrid", precision="fp32", refine=2, dense_cap=None, tune=True, **kw):
+    def __init__(self, ns, kind="hybrid", precision="fp32", refine=1, dense_cap=None, tune=True, **kw):
         self.ns = [int(n) for n in ns]
         self.kind = kind
         self.groups = {}

@@ -37,7 +37,7 @@ class GpuSlot:
     page-locked host buffers on the slot's stream; the inverse itself is replayed from a hipGraph captured once per
     batch size, so a launch costs three enqueues and the completion is one event."""
 
-    def __init__(self, n, cap, precision="fp32", refine=2, max_batch=4, setup=None, **kw):
+    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=4, setup=None, **kw):
         from . import lib as L
         from .inverses import FastHybridHip
 
@@ -112,7 +112,7 @@ class GpuSlot:
 
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
-                 precision="fp32", refine=2, **kw):
+                 precision="fp32", refine=1, **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
         deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory)."""
         self.keys = list(keys)

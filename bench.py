@@ -126,7 +126,7 @@ def mixed_batch_leg(count=512, dense_cap=16):
     R = pkg("radon")
     sinos = [ph.add_noise(R.adrt(ph.ellipses(1000 + i, 1 << n)), 0.01, 2000 + i) for i, n in enumerate(ns)]
     t0 = time.time()
-    mb = B.MixedBatch(ns, precision="fp32", refine=2, dense_cap=dense_cap)
+    mb = B.MixedBatch(ns, precision="fp32", refine=1, dense_cap=dense_cap)
     setup_s = time.time() - t0
     mb.set_inputs(sinos)
     mb.capture()
@@ -188,7 +188,7 @@ def large_leg(n=11, batch=16, dense_cap=16):
     t0 = time.time()
     H64 = I.FastHybridHip(n, dense_cap=dense_cap, precision="fp64", max_batch=batch)
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=dense_cap, precision="fp32", max_batch=batch,
-                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=2)
+                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=1)
     setup_s = time.time() - t0
     Bn = make_sinos(n, batch, 5000, 6000, level=0.005)
     st = L.Stream()
@@ -253,7 +253,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--n", type=int, default=8)
     ap.add_argument("--dense-cap", type=int, default=16)
-    ap.add_argument("--refine", type=int, default=2, help="FP64 refinement steps at the end of the FP32 apply")
+    ap.add_argument("--refine", type=int, default=1,
+                    help="FP64 outer steps at the end of the FP32 apply (1: the final residual and update in FP64)")
     ap.add_argument("--plain-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)

@@ -131,8 +131,10 @@ def test_fp32_and_refinement(gpu):
                               refine=refine)
         x32 = H32.apply_batch(B)
         err[refine] = max(np.linalg.norm(x32[i] - x64[i]) / np.linalg.norm(x64[i]) for i in range(2))
-    assert err[0] < 1e-4
-    assert err[3] < 1e-6 and err[3] < err[0] / 10, err
+    # FP32 alone is ~5e-7 from FP64 at N = 128; three FP64 outer steps take about a decade off that (each contracts
+    # the FP32 part of the error by the outer iteration's rate)
+    assert err[0] < 1e-5
+    assert err[3] < 1e-6 and err[3] < err[0] / 5, err
 
 
 def test_graph_replay_matches_eager(gpu):
