@@ -57,6 +57,19 @@ __host__ __device__ inline void two_sum(cplx<T> a, cplx<T> b, cplx<T>& s, cplx<T
     two_sum(a.y, b.y, s.y, e.y);
 }
 
+// ---------------------------------------------------------------- XCD-grouped block numbering
+// The dispatcher deals consecutive hardware block ids to the device's 8 XCDs in turn, and each XCD has its own L2.
+// Launching xcd_grid(nblk) blocks and taking logical block xcd_block(blockIdx.x, nblk) gives every XCD a contiguous
+// range of logical blocks, so blocks that read the same data (one image's tiles, one frequency's rows) share an L2.
+// xcd_block returns -1 for the few padding blocks of the last range (they must return before any barrier use).
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_block(int hw, int nblk) {
+    const int per = (nblk + XCDS - 1) / XCDS;
+    const int lb = (hw % XCDS) * per + hw / XCDS;
+    return lb < nblk ? lb : -1;
+}
+inline unsigned xcd_grid(long long nblk) { return (unsigned)(((nblk + XCDS - 1) / XCDS) * XCDS); }
+
 // ---------------------------------------------------------------- error reporting (thread-local last error)
 void set_error(const std::string& msg);
 const char* last_error_cstr();

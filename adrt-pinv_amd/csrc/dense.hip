@@ -116,8 +116,6 @@ int launch_n(int n, int L, int SP, int n_dense, const int* dense_k, const c128* 
 // not alias the input: other blocks still read the rows of the frequency.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int MF_XCD = 8;  // XCDs per device
-
 // U k-steps (of 4) per load group: 8 (32 k values) once N >= 32, so a group's 32 MFMAs (about 2000 cycles) cover the
 // latency of the next group's loads; the rows stay in their storage type in registers until used.
 template <typename Tg, int U>
@@ -125,10 +123,8 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(int n, int L, int SP, c
                                                          const void* Yin, void* Zout, int gx, int gy, int nblk,
                                                          int wr) {
     const int N = 1 << n;
-    // logical block id: the hardware id's XCD (id % 8) selects a contiguous eighth of the logical ids
-    const int hw = blockIdx.x, per = (nblk + MF_XCD - 1) / MF_XCD;
-    const int lb = (hw % MF_XCD) * per + hw / MF_XCD;
-    if (lb >= nblk) return;  // (the kernel has no barrier)
+    const int lb = xcd_block(blockIdx.x, nblk);
+    if (lb < 0) return;  // (the kernel has no barrier)
     const int bx = lb % gx, by = (lb / gx) % gy, slot = lb / (gx * gy);
     const int k = dense_k[slot];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -204,14 +200,13 @@ int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c12
     const int gx = (N / 16 + 4 / wr - 1) / (4 / wr), gy = (SP + 16 * wr - 1) / (16 * wr);
     const long long nblk = (long long)gx * gy * n_dense;
     ADRT_REQUIRE(nblk <= 0x7fffffffLL, "dense_mfma: too many blocks");
-    const int per = (int)((nblk + MF_XCD - 1) / MF_XCD);
     // 8 k-steps per load group for complex64 rows; complex128 rows take 4 (the larger group's registers would cost
     // half the waves)
     if (N >= 32 && sizeof(Tg) == 4)
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(per * MF_XCD), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
                            gx, gy, (int)nblk, wr);
     else
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(per * MF_XCD), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
                            gx, gy, (int)nblk, wr);
     ADRT_TRY(launch_check("dense_mfma_kernel"));
     if (out != Z) {

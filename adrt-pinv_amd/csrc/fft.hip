@@ -8,8 +8,8 @@ namespace adrt {
 
 namespace {
 
-constexpr int FFT_THREADS = 256;
-constexpr size_t FFT_LDS_BUDGET = 64 * 1024;
+constexpr int FFT_THREADS = 512;
+constexpr size_t FFT_LDS_BUDGET = 128 * 1024;  // of the 160 KiB of LDS per CU
 
 __device__ __forceinline__ int brev_bits(int v, int bits) { return bits ? (int)(__brev((unsigned)v) >> (32 - bits)) : 0; }
 

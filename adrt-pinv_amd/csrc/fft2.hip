@@ -69,7 +69,7 @@ __device__ __forceinline__ double image_strip2(const double* x, int N, int q, in
 
 // MODE 0: pack image strips (forward), 1: pack data strips (forward), 2: inverse + quadrant split
 template <typename Tg, typename T, int L1, int C, int MODE>
-__global__ __launch_bounds__(L1* C) void fft2_kernel(int n, const void* src, void* dst, const cplx<T>* twg) {
+__global__ __launch_bounds__(L1* C) void fft2_kernel(int n, int nsino, const void* src, void* dst, const cplx<T>* twg) {
     constexpr int L = 32 * L1;
     constexpr int P = C + 1;  // LDS pitch per row of the exchange
     constexpr bool INV = MODE == 2;
@@ -80,8 +80,12 @@ __global__ __launch_bounds__(L1* C) void fft2_kernel(int n, const void* src, voi
     const int N = 1 << n;
     const int t = threadIdx.x;
     const int cl = t % C, h1 = t / C;
-    const int c = blockIdx.x * C + cl;
-    const int pair = blockIdx.y, s = blockIdx.z;
+    // blocks of one sinogram / image on one XCD: an image is read once per quadrant of each pair, from that L2
+    const int gx = N / C;
+    const int lb = xcd_block(blockIdx.x, gx * 2 * nsino);
+    if (lb < 0) return;
+    const int c = (lb % gx) * C + cl;
+    const int pair = (lb / gx) % 2, s = lb / (2 * gx);
     for (int i = t; i < L / 2; i += L1 * C) tw[i] = twg[i];
     __syncthreads();
     cplx<T> w32[16];
@@ -163,8 +167,9 @@ int launch_cfg(int n, const void* src, void* dst, int S, const void* tw, hipStre
     constexpr int L = 32 * L1;
     const size_t lds = sizeof(cplx<T>) * ((size_t)L * (C + 1) + L / 2);
     const int N = 1 << n;
-    hipLaunchKernelGGL((fft2_kernel<Tg, T, L1, C, MODE>), dim3(N / C, 2, S), dim3(L1 * C), lds, st, n, src, dst,
-                       (const cplx<T>*)tw);
+    ADRT_REQUIRE((long long)(N / C) * 2 * S <= 0x7fffffffLL, "fft2: too many blocks");
+    hipLaunchKernelGGL((fft2_kernel<Tg, T, L1, C, MODE>), dim3(xcd_grid((long long)(N / C) * 2 * S)), dim3(L1 * C), lds,
+                       st, n, S, src, dst, (const cplx<T>*)tw);
     return launch_check("fft2_kernel");
 }
 

@@ -91,6 +91,7 @@ constexpr int G1_THREADS = 512;
 
 struct G1Args {
     int n, deg, mode;
+    int S;             // images
     const double* y;   // right-hand side (S, N, N)
     const double* x;   // outer iterate (mode 2)
     const double* xp;  // outer previous iterate (mode 2, may be null = 0)
@@ -158,12 +159,20 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     Cell4<T>* X = reinterpret_cast<Cell4<T>*>(g1_smem);  // two buffers of (CP, CP) cells
     const int N = 1 << A.n;
     const int H = A.deg - 1;  // <= HALO
-    const int s = blockIdx.z;
-    const int gi0 = blockIdx.y * TILE - HALO, gj0 = blockIdx.x * TILE - HALO;
+    // the tiles of one image on one XCD: neighbouring tiles' halos overlap, and are then read from that L2
+    const int nt = (N + TILE - 1) / TILE;
+    const int lb = xcd_block(blockIdx.x, nt * nt * A.S);
+    if (lb < 0) return;
+    const int s = lb / (nt * nt);
+    const int gi0 = ((lb / nt) % nt) * TILE - HALO, gj0 = (lb % nt) * TILE - HALO;
     const double* ys = A.y + (size_t)s * N * N;
     const T a = (T)(2.0 / 20.0);
     for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = Cell4<T>{{(T)0, (T)0, (T)0, (T)0}};
     __syncthreads();
+    // float carries the increment d_k = x_k - x_(k-1) in xq and a y in yv: x_(k+1) = x_k + d_(k+1) with
+    // d_(k+1) = w a (y - G_1 x_k) + (w - 1) d_k (the same recurrence, fewer instructions); double keeps the reference's
+    // x + w (a r) + (w - 1)(x - x_prev) (fasthybrid.py:114-126) operation for operation.
+    constexpr bool INCR = std::is_same<T, float>::value;
     T yv[BPT][4], xc[BPT][4], xq[BPT][4];
     unsigned inm[BPT];
     int lds[BPT];  // LDS cell index of the thread's cells
@@ -182,6 +191,10 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             yv[e][c] = in ? (T)ys[(size_t)gi * N + gj] : (T)0;
             xc[e][c] = a * yv[e][c];
             xq[e][c] = (T)0;
+            if constexpr (INCR) {
+                yv[e][c] = xc[e][c];  // a y
+                xq[e][c] = xc[e][c];  // the first increment x_0 - 0
+            }
         }
         if (b < NB) X[lds[e]] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
     }
@@ -231,10 +244,17 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             g[3] = eight * x11 + (x00 + two * x10 + dn0) + (x00 + two * x01 + rt0);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const T xo = xc[e][c];
-                const T xn = ((inm[e] >> c) & 1) ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : (T)0;
-                xq[e][c] = xo;
-                xc[e][c] = xn;
+                const bool in = (inm[e] >> c) & 1;
+                if constexpr (INCR) {
+                    const T d = om * fma(-a, g[c], yv[e][c]) + om1 * xq[e][c];
+                    xq[e][c] = in ? d : (T)0;
+                    xc[e][c] = in ? xc[e][c] + d : (T)0;
+                } else {
+                    const T xo = xc[e][c];
+                    const T xn = in ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : (T)0;
+                    xq[e][c] = xo;
+                    xc[e][c] = xn;
+                }
             }
             Xw[o] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
         }
@@ -270,7 +290,9 @@ int launch_g1_cfg(const G1Args& A, int S, hipStream_t st) {
     const int N = 1 << A.n;
     const int nt = (N + TILE - 1) / TILE;
     constexpr size_t lds = G1Cfg<T, TILE, HALO>::LDS;
-    hipLaunchKernelGGL((g1_fused_kernel<T, TILE, HALO>), dim3(nt, nt, S), dim3(G1_THREADS), lds, st, A);
+    ADRT_REQUIRE((long long)nt * nt * S <= 0x7fffffffLL, "g1_fused: too many blocks");
+    hipLaunchKernelGGL((g1_fused_kernel<T, TILE, HALO>), dim3(xcd_grid((long long)nt * nt * S)), dim3(G1_THREADS), lds, st,
+                       A);
     return launch_check("g1_fused_kernel");
 }
 
@@ -303,6 +325,7 @@ int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, 
     A.w1 = w1;
     A.w0 = w0;
     A.out = out;
+    A.S = S;
     const double c = 12.0 / 20.0;
     double om = 1.0;
     for (int k = 1; k < deg; ++k) {
