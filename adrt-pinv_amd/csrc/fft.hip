@@ -215,6 +215,7 @@ int launch_pack_fft(int n, int mode, const double* src, void* Y, int S, int prec
     ADRT_REQUIRE(mode == 0 || mode == 1, "pack_fft: mode must be 0 (image) or 1 (data)");
     if (S <= 0) return OK;
     if (fft2_supported(n)) return launch_fft2(n, mode, src, Y, S, prec, tw, st);
+    if (fft3_supported(n)) return launch_fft3(n, mode, src, Y, S, prec, tw, st);
     if (prec == PREC_F64) return pack_impl<double, double>(n, mode, src, Y, S, tw, st);
     return pack_impl<float, float>(n, mode, src, Y, S, tw, st);
 }
@@ -223,6 +224,7 @@ int launch_ifft_unpack(int n, const void* Z, void* Sq, int S, int prec, const vo
     ADRT_REQUIRE(n >= 1 && n <= 11, "ifft_unpack: need 1 <= n <= 11");
     if (S <= 0) return OK;
     if (fft2_supported(n)) return launch_fft2(n, 2, Z, Sq, S, prec, tw, st);
+    if (fft3_supported(n)) return launch_fft3(n, 2, Z, Sq, S, prec, tw, st);
     if (prec == PREC_F64) return ifft_impl<double, double>(n, Z, Sq, S, tw, st);
     return ifft_impl<float, float>(n, Z, Sq, S, tw, st);
 }

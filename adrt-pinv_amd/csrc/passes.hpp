@@ -22,6 +22,9 @@ int launch_combine(int n, const void* Sq, int prec, const double* y, double* out
 // Two-step register FFT for 6 <= n <= 9 (fft2.hip); mode 0 / 1 as launch_pack_fft, mode 2 as launch_ifft_unpack.
 bool fft2_supported(int n);
 int launch_fft2(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st);
+// Three-step register FFT for n = 10, 11 (fft3.hip), modes as launch_fft2.
+bool fft3_supported(int n);
+int launch_fft3(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st);
 
 // Twiddle table exp(sign * 2 pi i m / L), m < L/2, in the precision's type (host builds it once per plan).
 size_t twiddle_bytes(int n, int prec);

@@ -94,3 +94,33 @@ def test_lstsq_cheb_recovers_n256(gpu):
     x = pkg("phantom").ellipses(7, N)
     out = pkg("lsq").lstsq_cheb(radon.adrt(n, x), lam_min=0.95 * 88.69, tol=1e-9)
     assert np.linalg.norm(out - x) <= 1e-7 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("n", [9, 10, 11])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gram_and_backprojection_large_n(gpu, n, dtype):
+    """At N = 512 .. 2048 (the offset FFTs of every size: two-step at n = 9, three-step at n = 10, 11) the frequency
+    path equals the spatial one: gram(a) = R^T R a and the normal plan's backprojection = R^T b, against the GPU level
+    kernels (bit-exact against the oracle's ADRT, test_gpu_adrt.py)."""
+    import ctypes
+
+    R = pkg("radon")
+    lsq = pkg("lsq")
+    N = 1 << n
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((N, N))
+    ref = R.adrt_T(R.adrt(a))
+    tol = 1e-11 if dtype == np.float64 else 2e-5
+    out = lsq.gram(a.astype(dtype))
+    assert out.dtype == dtype
+    assert np.abs(out - ref).max() <= tol * np.abs(ref).max()
+    b = rng.standard_normal((1, 4, 2 * N - 1, N))
+    P = lsq._plan(n, "fp64" if dtype == np.float64 else "fp32")
+    L = gpu
+    lib = L.load()
+    L.check(lib.adrt_memcpy(P.d_sino.ptr, b.ctypes.data, b.nbytes, 1, None), "h2d")
+    L.check(lib.adrt_plan_backproject(P._plan, P.d_sino.ptr, P.d_out.ptr, 1, None), "backproject")
+    y = np.empty((N, N))
+    L.check(lib.adrt_memcpy(y.ctypes.data, P.d_out.ptr, y.nbytes, 2, None), "d2h")
+    yref = R.adrt_T(b[0])
+    assert np.abs(y - yref).max() <= tol * np.abs(yref).max()
