@@ -112,7 +112,7 @@ class GpuSlot:
 
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
-                 precision="fp32", refine=1, **kw):
+                 precision="fp32", refine=1, tune=True, **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
         deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory)."""
         self.keys = list(keys)
@@ -141,6 +141,14 @@ class Server:
                 self.slots.append(s)
         self.device_bytes = used
         self.served = {k: any((s.n, s.cap) == k for s in self.slots) for k in self.keys}
+        if tune and slot_factory is None:
+            # the row kernel's mapping per N, timed on the first slot of that N at its batch depth (before the graphs
+            # are captured, which bake the mapping in)
+            from . import autotune
+
+            for n in sorted({s.n for s in self.slots}):
+                s = next(x for x in self.slots if x.n == n)
+                autotune.tune(s.plan, s.max_batch, s.stream, persist=False)
         for s in self.slots:
             if hasattr(s, "prepare"):
                 s.prepare()

@@ -147,7 +147,7 @@ def mixed_batch_leg(count=512, dense_cap=16):
             "setup_s": setup_s, "row_mapping_loge": tuned, "dense_cap": dense_cap}
 
 
-SERVE_BATCH = {7: 16, 8: 8, 9: 4, 10: 2}  # sinograms per launch by log2 N (small N batch deeper)
+SERVE_BATCH = {7: 16, 8: 16, 9: 8, 10: 8}  # sinograms per launch by log2 N
 
 
 def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), slots_per_key=2):
