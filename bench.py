@@ -189,6 +189,7 @@ def large_leg(n=11, batch=16, dense_cap=16):
     H64 = I.FastHybridHip(n, dense_cap=dense_cap, precision="fp64", max_batch=batch)
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=dense_cap, precision="fp32", max_batch=batch,
                           outer=(H64.lo, H64.hi, H64.outer_deg), refine=1)
+    pkg("autotune").tune(H32, batch, persist=False)
     setup_s = time.time() - t0
     Bn = make_sinos(n, batch, 5000, 6000, level=0.005)
     st = L.Stream()
