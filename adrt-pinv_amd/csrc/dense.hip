@@ -176,26 +176,10 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(int n, int L, int SP, c
     }
 }
 
-// copies the dense rows of Y2 back into Y
-template <typename Tg>
-__global__ void dense_copy_kernel(int n, int L, int SP, const int* dense_k, const void* src, void* dst) {
-    const int N = 1 << n;
-    const int slot = blockIdx.y;
-    const int k = dense_k[slot];
-    const cplx<Tg>* S = reinterpret_cast<const cplx<Tg>*>(src);
-    cplx<Tg>* D = reinterpret_cast<cplx<Tg>*>(dst);
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)SP * N;
-         e += (long long)gridDim.x * blockDim.x) {
-        const size_t o = ((size_t)(e / N) * L + k) * N + (e % N);
-        D[o] = S[o];
-    }
-}
-
 template <typename Tg>
 int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                void* tmp, hipStream_t st) {
+                hipStream_t st) {
     const int N = 1 << n;
-    void* out = (Z == Y) ? tmp : Z;
     const int wr = SP > 32 ? 4 : (SP > 16 ? 2 : 1);  // waves of a block along the rows
     const int gx = (N / 16 + 4 / wr - 1) / (4 / wr), gy = (SP + 16 * wr - 1) / (16 * wr);
     const long long nblk = (long long)gx * gy * n_dense;
@@ -203,31 +187,24 @@ int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c12
     // 8 k-steps per load group for complex64 rows; complex128 rows take 4 (the larger group's registers would cost
     // half the waves)
     if (N >= 32 && sizeof(Tg) == 4)
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, Z,
                            gx, gy, (int)nblk, wr);
     else
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, out,
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, Z,
                            gx, gy, (int)nblk, wr);
-    ADRT_TRY(launch_check("dense_mfma_kernel"));
-    if (out != Z) {
-        long long cnt = (long long)SP * N;
-        unsigned gx = (unsigned)((cnt + 255) / 256 < 4096 ? (cnt + 255) / 256 : 4096);
-        hipLaunchKernelGGL(dense_copy_kernel<Tg>, dim3(gx, n_dense), dim3(256), 0, st, n, L, SP, dense_k, out, Z);
-        ADRT_TRY(launch_check("dense_copy_kernel"));
-    }
-    return OK;
+    return launch_check("dense_mfma_kernel");
 }
 
 }  // namespace
 
-// Z_r = W_slot Y_r on the FP64 matrix cores (requires N >= 16 and, when Z aliases Y, a row buffer tmp of Y's size).
+// Z_r = W_slot Y_r on the FP64 matrix cores (requires N >= 16 and Z distinct from Y).
 int launch_dense_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                      void* tmp, int prec, hipStream_t st) {
+                      int prec, hipStream_t st) {
     if (n_dense <= 0 || SP <= 0) return OK;
     ADRT_REQUIRE(n >= 4 && n <= 11, "dense_mfma: need 4 <= n <= 11");
-    ADRT_REQUIRE(Z != Y || tmp, "dense_mfma: in-place use needs a scratch row buffer");
-    if (prec == PREC_F64) return launch_mfma<double>(n, L, SP, n_dense, dense_k, WT, Y, Z, tmp, st);
-    return launch_mfma<float>(n, L, SP, n_dense, dense_k, WT, Y, Z, tmp, st);
+    ADRT_REQUIRE(Z != Y, "dense_mfma: the output must not alias the input rows");
+    if (prec == PREC_F64) return launch_mfma<double>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    return launch_mfma<float>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
 }
 
 // Z_r = W_slot Y_r for the SP rows of every dense frequency (rows sp * L + k); Z may alias Y.

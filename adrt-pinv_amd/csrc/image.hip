@@ -1,5 +1,6 @@
 // Image-space kernels of the outer iteration (fastops.py:247-258) and the level-1 preconditioners
-// (fasthybrid.py:81-130): Chebyshev updates, the G_1 stencil step, the lorr projector.  All in double.
+// (fasthybrid.py:81-130): Chebyshev updates, the G_1 stencil step and the fused g1_inv, the lorr projector.  Images
+// are double; the fused g1_inv iterates in float inside the FP32 outer steps of an FP32 plan.
 #include <type_traits>
 
 #include "passes.hpp"

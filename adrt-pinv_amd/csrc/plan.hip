@@ -60,11 +60,13 @@ void free_plan(Plan* p) {
     delete p;
 }
 
-int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t st, void* out = nullptr,
+// The row ops on the plan's rows Y, written out of place to `out` (the dense GEMM's blocks read all rows of their
+// frequency while others write).
+int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t st, void* out,
               const c128* WT = nullptr) {
     RowsArgs A{};
     A.Y = p->Y;
-    A.Z = out ? out : p->Y;
+    A.Z = out;
     A.rows = (long long)S * 2 * p->L;
     A.in_stride = A.out_stride = p->N;
     A.n = p->d.n;
@@ -98,7 +100,7 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         // bits alone and in a batch.  (A last-bit difference of an FP64 pass would otherwise reach an FP32 rounding
         // of the residual, whose cancellation magnifies it, and grow to the FP32 level.)
         if (p->d.n >= 4 && (p->d.precision == PREC_F32 || 2 * S > 4)) {
-            return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, p->Y2, prec, st);
+            return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
         }
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
     }

@@ -55,7 +55,7 @@ int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, con
                       int prec, hipStream_t st);
 
 int launch_dense_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                      void* tmp, int prec, hipStream_t st);
+                      int prec, hipStream_t st);
 
 // Validates an op list against n; returns OK or E_INVALID with the error set.
 int check_ops(int n, const RowOp* ops, int nops);

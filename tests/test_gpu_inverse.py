@@ -158,3 +158,23 @@ def test_graph_replay_matches_eager(gpu):
         got = dx.download(st)
         assert relmax(got, eager) < 1e-13, rep
         H.apply(d["b"])  # eager work between replays
+
+
+def test_fp32_batch_headline_config(gpu):
+    """The benchmarked configuration at a smaller batch: N = 256, dense_cap 16, ellipse phantoms + 1% noise, the FP32
+    plan (TwoSum rows, one FP64 refinement step) within 1e-5 of the FP64 plan for every sinogram, and a batch equal to
+    its sinograms run one at a time (same bits)."""
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    n, S, cap = 8, 6, 16
+    N = 1 << n
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(1000 + i, N)), 0.01, 2000 + i) for i in range(S)])
+    H64 = I.FastHybridHip(n, dense_cap=cap, max_batch=S)
+    x64 = H64.apply_batch(B)
+    H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=S,
+                          outer=(H64.lo, H64.hi, H64.outer_deg), refine=1)
+    x32 = H32.apply_batch(B)
+    for i in range(S):
+        assert np.linalg.norm(x32[i] - x64[i]) / np.linalg.norm(x64[i]) < 1e-5, i
+    assert np.array_equal(x32[2], H32.apply(B[2]))
