@@ -55,6 +55,8 @@ void report(int st, const char* what) {
 int bfly_rows_impl(int n, int K, const void* z, void* X, int m_lo, int m_hi, int transpose) {
     ADRT_REQUIRE(n >= 1 && n <= 11, "bfly_rows: need 1 <= n <= 11");
     ADRT_REQUIRE(K >= 0, "bfly_rows: K must be >= 0");
+    ADRT_REQUIRE(1 <= m_lo && m_lo <= m_hi && m_hi <= n, "bfly_rows: need 1 <= m_lo <= m_hi <= n");
+    ADRT_REQUIRE(transpose == 0 || transpose == 1, "bfly_rows: transpose must be 0 or 1");
     if (K == 0) return OK;
     ADRT_REQUIRE(z && X, "bfly_rows: null pointer");
     const int N = 1 << n;

@@ -1,0 +1,76 @@
+"""Argument errors of the C ABI, on the CPU: every entry point validates its arguments before any HIP call and
+returns -1 with a message in adrt_last_error() (the reference's C functions do not validate: bad arguments there are
+undefined behaviour, cpp/oblique.cpp:56-132)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import pkg
+
+
+@pytest.fixture(scope="module")
+def L():
+    return pkg("lib")
+
+
+def err(L):
+    m = L.load().adrt_last_error()
+    return m.decode() if m else ""
+
+
+def desc(L, **kw):
+    d = L.PlanDesc()
+    z = np.ones(16, dtype=np.complex128)
+    deg = np.ones(16, dtype=np.int32)
+    ac = np.ones(16)
+    d.kind, d.n, d.precision, d.precond, d.g1_deg, d.outer_deg = L.KIND_HYBRID, 3, 0, 1, 14, 10
+    d.lo, d.hi, d.tau = 1.0, 2.0, 1.6
+    d.z, d.deg, d.a, d.c = L.ptr(z), L.ptr(deg), L.ptr(ac), L.ptr(ac)
+    d.max_batch = 1
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d, (z, deg, ac)
+
+
+@pytest.mark.parametrize("field,value,word", [("kind", 7, "kind"), ("n", 12, "n"), ("max_batch", 0, "max_batch"),
+                                               ("precision", 5, "precision"), ("g1_deg", 0, "g1_deg"),
+                                               ("outer_deg", 0, "outer_deg"), ("hi", -1.0, "hi")])
+def test_plan_create_rejects(L, field, value, word):
+    d, keep = desc(L, **{field: value})
+    h = L.P()
+    assert L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)) == -1
+    assert h.value is None and word in err(L)
+
+
+def test_normal_plan_rejects_dense_rows(L):
+    d, keep = desc(L, kind=L.KIND_NORMAL, n_dense=2)
+    h = L.P()
+    assert L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)) == -1
+    assert "dense" in err(L)
+
+
+def test_row_kernels_reject_bad_levels(L):
+    lib = L.load()
+    z = np.ones(4, dtype=np.complex128)
+    X = np.zeros((4, 8), dtype=np.complex128)
+    assert lib.adrt_bfly_rows(3, 4, L.ptr(z), L.ptr(X), 2, 1, 0) == -1 and "m_lo" in err(L)
+    assert lib.adrt_bfly_rows(3, 4, L.ptr(z), L.ptr(X), 1, 4, 0) == -1
+    assert lib.adrt_bfly_rows(3, 4, L.ptr(z), L.ptr(X), 1, 3, 2) == -1 and "transpose" in err(L)
+    deg = np.ones(4, dtype=np.int32)
+    ac = np.ones(4)
+    assert lib.adrt_cheb_rows(3, 4, L.ptr(z), L.ptr(deg), L.ptr(ac), L.ptr(ac), 1.0, L.ptr(X), L.ptr(X), 0, 3,
+                              None) == -1
+    assert lib.adrt_bfly_rows(12, 4, L.ptr(z), L.ptr(X), 1, 3, 0) == -1 and "n" in err(L)
+
+
+def test_memory_and_plan_calls_reject(L):
+    lib = L.load()
+    buf = np.zeros(4)
+    assert lib.adrt_memcpy(L.ptr(buf), L.ptr(buf), 32, 9, None) == -1 and "kind" in err(L)
+    assert lib.adrt_host_malloc(None, 16) == -1
+    assert lib.adrt_plan_set_outer(None, 1.0, 2.0, 3) == -1 and "null plan" in err(L)
+    assert lib.adrt_plan_apply(None, None, None, 1, None) == -1
+    h = L.P()
+    assert lib.adrt_rows_plan_create(ctypes.byref(h), 3, 16, None, None, None, None, 1.0, 0, None, None, 0) == -1

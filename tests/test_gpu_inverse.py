@@ -178,3 +178,17 @@ def test_fp32_batch_headline_config(gpu):
     for i in range(S):
         assert np.linalg.norm(x32[i] - x64[i]) / np.linalg.norm(x64[i]) < 1e-5, i
     assert np.array_equal(x32[2], H32.apply(B[2]))
+
+
+def test_plan_batch_limits(gpu):
+    """A batch above the plan's max_batch is an error, not an overrun; an empty batch is a no-op."""
+    L = gpu
+    I = pkg("inverses")
+    d = golden("inverse_n4_cap-1")
+    H = I.FastHybridHip(4, lam_max=d["h_lam_max"], outer=(float(d["h_lo"]), float(d["h_hi"]), int(d["h_D"])),
+                        max_batch=2)
+    db = L.DeviceArray((3, 4, 31, 16))
+    dx = L.DeviceArray((3, 16, 16))
+    with pytest.raises(L.HipError, match="max_batch"):
+        H.apply_device(db, dx, 3)
+    H.apply_device(db, dx, 0)
