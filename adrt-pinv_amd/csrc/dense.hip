@@ -6,6 +6,7 @@
 // 2S rows of frequency k (both packed pairs of every sinogram) by W_k: out_r = W_k in_r.  One block owns TR rows of
 // one frequency and all N outputs, so it may write its rows in place; W_k^T is streamed from L2/HBM (coalesced over
 // the outputs) and reused by the block's TR rows, whose inputs are staged through LDS in chunks of TJ columns.
+#include "dense.hpp"
 #include "rows.hpp"
 
 namespace adrt {
@@ -100,80 +101,25 @@ int launch_n(int n, int L, int SP, int n_dense, const int* dense_k, const c128* 
 }
 
 // ---------------------------------------------------------------- MFMA version (FP64 matrix cores)
-// C[r][i] = sum_j Y[r][j] WT[j][i] for the SP rows r of one dense frequency, complex, as four real
-// v_mfma_f64_16x16x4 products per k-step (C_re += Yr Wr - Yi Wi, C_im += Yr Wi + Yi Wr).  Operand layouts of the
-// 16x16x4 f64 instruction (checked on gfx950): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
-// D[4 r + lane / 16][lane % 16] in accumulator element r.
-//
-// Each wave owns one 16 x 16 output tile (16 rows of one frequency, 16 outputs) and streams its operands from
-// L2 / HBM straight into the MFMA layout, U k-steps per group with the next group's loads in flight while the
-// current group's 4 U MFMAs run.  A block is four waves: WR = min(4, rows / 16) of them on consecutive row tiles
-// and 4 / WR on consecutive 16-output column tiles, so with many rows the block's W^T columns are fetched once into
-// L1 and shared, and with few rows (a single sinogram) every wave still has rows to work on.  At N = 256 with 128 rows and 33 dense frequencies that is
-// 4224 waves of equal work, about four per SIMD, so the tail costs one wave's time.  Blocks are numbered so that the
-// blocks of one frequency run on one XCD (the dispatcher deals consecutive blocks to the 8 XCDs in turn): the
-// frequency's W^T then comes from HBM once per XCD and from that XCD's L2 for its other row groups.  The output must
-// not alias the input: other blocks still read the rows of the frequency.
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// U k-steps (of 4) per load group: 8 (32 k values) once N >= 32, so a group's 32 MFMAs (about 2000 cycles) cover the
-// latency of the next group's loads; the rows stay in their storage type in registers until used.
+// Each wave owns one 16 x 16 output tile (dense.hpp: dense::tile).  A block is four waves: WR = min(4, rows / 16) of
+// them on consecutive row tiles and 4 / WR on consecutive 16-output column tiles, so with many rows the block's W^T
+// columns are fetched once into L1 and shared, and with few rows (a single sinogram) every wave still has rows to
+// work on.  At N = 256 with 128 rows and 33 dense frequencies that is 4224 waves of equal work, about four per SIMD,
+// so the tail costs one wave's time.  Blocks are numbered so that the blocks of one frequency run on one XCD (the
+// dispatcher deals consecutive blocks to the 8 XCDs in turn): the frequency's W^T then comes from HBM once per XCD
+// and from that XCD's L2 for its other row groups.
 template <typename Tg, int U>
-__global__ __launch_bounds__(256) void dense_mfma_kernel(int n, int L, int SP, const int* dense_k, const c128* WT,
-                                                         const void* Yin, void* Zout, int gx, int gy, int nblk,
-                                                         int wr) {
+__global__ __launch_bounds__(256) void dense_mfma_kernel(int n, DenseArgs D, const void* Yin, void* Zout, int gx,
+                                                         int gy, int nblk, int wr) {
     const int N = 1 << n;
     const int lb = xcd_block(blockIdx.x, nblk);
     if (lb < 0) return;  // (the kernel has no barrier)
     const int bx = lb % gx, by = (lb / gx) % gy, slot = lb / (gx * gy);
-    const int k = dense_k[slot];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = (by * wr + wave % wr) * 16;
     const int c0 = (bx * (4 / wr) + wave / wr) * 16;
-    if (r0 >= SP || c0 >= N) return;
-    const int li = lane & 15, kq = lane >> 4;
-    const c128* W = WT + (size_t)slot * N * N + c0 + li;  // B[kq][li] of k-step kb: W[(kb + kq) * N]
-    const int row = r0 + li;
-    const bool rv = row < SP;
-    const cplx<Tg>* Arow = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv ? row : 0) * L + k) * N + kq;
-    const cplx<Tg> zero = mk((Tg)0, (Tg)0);
-    v4d cre = v4d{0.0, 0.0, 0.0, 0.0}, cim = v4d{0.0, 0.0, 0.0, 0.0};
-    cplx<Tg> a[U], an[U];
-    c128 b[U], bn[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        a[u] = rv ? Arow[4 * u] : zero;
-        b[u] = W[(size_t)(4 * u + kq) * N];
-    }
-    for (int kb = 0; kb < N; kb += 4 * U) {
-        const int kn = kb + 4 * U;
-        if (kn < N) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                an[u] = rv ? Arow[kn + 4 * u] : zero;
-                bn[u] = W[(size_t)(kn + 4 * u + kq) * N];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const double ar = (double)a[u].x, ai = (double)a[u].y;
-            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].x, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, b[u].y, cre, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].y, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[u].x, cim, 0, 0, 0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            a[u] = an[u];
-            b[u] = bn[u];
-        }
-    }
-    cplx<Tg>* Z = reinterpret_cast<cplx<Tg>*>(Zout);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int sp = r0 + 4 * e + kq;
-        if (sp < SP) Z[((size_t)sp * L + k) * N + c0 + li] = cast<Tg>(mk(cre[e], cim[e]));
-    }
+    if (r0 >= D.SP || c0 >= N) return;
+    dense::tile<Tg, U>(n, D, slot, r0, c0, lane, Yin, Zout);
 }
 
 template <typename Tg>
@@ -184,14 +130,20 @@ int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c12
     const int gx = (N / 16 + 4 / wr - 1) / (4 / wr), gy = (SP + 16 * wr - 1) / (16 * wr);
     const long long nblk = (long long)gx * gy * n_dense;
     ADRT_REQUIRE(nblk <= 0x7fffffffLL, "dense_mfma: too many blocks");
+    DenseArgs D;
+    D.dense_k = dense_k;
+    D.WT = WT;
+    D.n_dense = n_dense;
+    D.L = L;
+    D.SP = SP;
     // 8 k-steps per load group for complex64 rows; complex128 rows take 4 (the larger group's registers would cost
     // half the waves)
     if (N >= 32 && sizeof(Tg) == 4)
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, Z,
-                           gx, gy, (int)nblk, wr);
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, D, Y, Z, gx, gy,
+                           (int)nblk, wr);
     else
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, L, SP, dense_k, WT, Y, Z,
-                           gx, gy, (int)nblk, wr);
+        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, D, Y, Z, gx, gy,
+                           (int)nblk, wr);
     return launch_check("dense_mfma_kernel");
 }
 

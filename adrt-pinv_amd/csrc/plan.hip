@@ -90,18 +90,28 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         A.korder = p->korder_nd;
         A.nk = p->L - p->d.n_dense;
         A.skip_dense = 1;
-        if (A.nk > 0) {
-            if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
-            ADRT_TRY(launch_rows(A, prec, st));
-            if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
-        }
-        // The MFMA tile (64 rows) pays off once several sinograms share W.  Every pass of an FP32 plan takes it (its
+        // The MFMA tile (16 rows) pays off once several sinograms share W.  Every pass of an FP32 plan takes it (its
         // FP64 passes too): its per-row arithmetic does not depend on the batch size, so a sinogram gives the same
         // bits alone and in a batch.  (A last-bit difference of an FP64 pass would otherwise reach an FP32 rounding
-        // of the residual, whose cancellation magnifies it, and grow to the FP32 level.)
-        if (p->d.n >= 4 && (p->d.precision == PREC_F32 || 2 * S > 4)) {
-            return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
+        // of the residual, whose cancellation magnifies it, and grow to the FP32 level.)  The register row kernel
+        // takes the MFMA tiles in extra waves of its own launch (the tiles use the matrix cores while the rows use
+        // the vector ALUs); otherwise they are a launch of their own.
+        const bool mfma = p->d.n >= 4 && (p->d.precision == PREC_F32 || 2 * S > 4);
+        int dense_done = 0;
+        if (mfma && A.nk > 0) {
+            A.dn.dense_k = p->dense_k;
+            A.dn.WT = WT;
+            A.dn.n_dense = p->d.n_dense;
+            A.dn.L = p->L;
+            A.dn.SP = 2 * S;
         }
+        if (A.nk > 0) {
+            if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
+            ADRT_TRY(launch_rows(A, prec, st, &dense_done));
+            if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
+        }
+        if (dense_done) return OK;
+        if (mfma) return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
     }
     if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));

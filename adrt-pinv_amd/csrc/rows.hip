@@ -308,16 +308,22 @@ int check_ops(int n, const RowOp* ops, int nops) {
     return OK;
 }
 
-int launch_rows(const RowsArgs& A, int prec, hipStream_t s) {
+int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
+    if (dense_done) *dense_done = 0;
     ADRT_REQUIRE(A.n >= 1 && A.n <= 11 && A.N == (1 << A.n), "rows: need 1 <= n <= 11 and N = 2^n");
     ADRT_TRY(check_ops(A.n, A.ops, A.nops));
     ADRT_REQUIRE(A.period >= 1, "rows: period must be positive");
+    ADRT_REQUIRE(A.dn.n_dense == 0 || (A.skip_dense && A.Z != A.Y && dense_done),
+                 "rows: dense tiles need skip_dense, distinct input and output rows and a completion flag");
     if (A.loge >= 0 && rows_config_get(A.n, prec) >= 0) {
         const int cfg = rows_config_get(A.n, prec);
         const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));
         if (loge > 0) {
             const int st = launch_rows_reg(A, prec, loge, s);
-            if (st == OK) return OK;
+            if (st == OK) {
+                if (dense_done) *dense_done = A.dn.n_dense > 0;
+                return OK;
+            }
             if (st != E_INVALID) return st;
         }
     }

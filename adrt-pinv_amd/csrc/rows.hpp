@@ -8,7 +8,7 @@
 // pairs in reverse level order and returns the natural order, so a B..BT sandwich (the Gram M = B^H B inside the
 // Chebyshev filter) never permutes data.
 #pragma once
-#include "common.hpp"
+#include "dense.hpp"
 
 namespace adrt {
 
@@ -38,14 +38,21 @@ struct RowsArgs {
     int skip_dense = 0;  // rows of dense frequencies are left untouched (a dense GEMM handles them)
     int loge = 0;  // register kernel elements per lane (log2); 0 = default for (n, precision), -1 = generic kernel
     int comp = 0;  // FP32 only: compensated (TwoSum) accumulation of the Chebyshev iterate
+    // dense rows of the pass for the register kernel to compute as MFMA tiles beside the rows (n_dense = 0: none);
+    // requires skip_dense and Z distinct from Y
+    DenseArgs dn{};
 };
 
 // Host launcher (rows.hip).  prec selects the compute type of the butterflies and Chebyshev steps; the global row
-// type is complex128 for PREC_F64 and complex64 for PREC_F32.
-int launch_rows(const RowsArgs& args, int prec, hipStream_t stream);
+// type is complex128 for PREC_F64 and complex64 for PREC_F32.  When args.dn names dense rows, *dense_done is set to 1
+// if the register kernel computed them and 0 if the caller still has to (the generic kernel ignores args.dn).
+int launch_rows(const RowsArgs& args, int prec, hipStream_t stream, int* dense_done = nullptr);
 
 // Register-resident kernel (rows_reg.hip); E_INVALID if the op list or size needs the generic kernel.
 int launch_rows_reg(const RowsArgs& args, int prec, int loge, hipStream_t stream);
+// Dense waves the register kernel adds for args.dn: one per SIMD of the device at most (the row waves keep the
+// others), never more than the pass has tiles.
+int dense_workers(int n, const DenseArgs& dn);
 int default_loge(int n, int prec);
 int rows_config_get(int n, int prec);
 int rows_config_set(int n, int prec, int loge);

@@ -2,6 +2,8 @@
 // instantiation units rows_reg_{d,s,c}{1,2}.hip.
 #include "rows_reg.hpp"
 
+#include <atomic>
+
 namespace adrt {
 
 // Decodes a reference op list into the kernel's macro-ops; returns false if the list needs the generic kernel.  The
@@ -44,6 +46,21 @@ int default_loge(int n, int prec) {
     if (n < 5 || n > 11) return 0;
     if (prec == PREC_F32) return 3;
     return 2;
+}
+
+int dense_workers(int n, const DenseArgs& dn) {
+    static std::atomic<int> simds[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int sm = simds[dev].load(std::memory_order_relaxed);
+    if (sm == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        sm = 4 * cus;
+        simds[dev].store(sm, std::memory_order_relaxed);
+    }
+    const long long T = dense::tiles(n, dn);
+    return (int)(T < sm ? T : sm);
 }
 
 int launch_rows_reg(const RowsArgs& A, int prec, int loge, hipStream_t s) {

@@ -19,6 +19,11 @@
 // Rows of one block run the same macro-op program; P rows with different degrees iterate to the block's largest
 // degree with the update predicated off, so every barrier is block-uniform.  The plan maps blocks to rows of one
 // frequency (equal degree) in descending-degree order.
+//
+// A pass with dense rows (RowsArgs::dn) puts its dense GEMM tiles in the same launch: the leading blocks are dense
+// workers, one wave per SIMD at most, each taking a contiguous share of the 16 x 16 MFMA tiles (dense.hpp), while
+// the row blocks behind them fill the remaining wave slots.  The tiles' loads and matrix-core work then overlap the
+// rows' vector work instead of running as a launch of their own after it.
 #pragma once
 #include "rows.hpp"
 
@@ -69,6 +74,9 @@ struct Cfg {
     static constexpr int RPB = NT / G;
     static constexpr int NPASS = (LOGN + LOGE - 1) / LOGE;
     static constexpr int ROW = N + (N >> LOGE);  // padded row length in LDS (also holds the z^t table, N/2 + 1)
+    // waves per SIMD the register allocation must allow: four for float rows of E = 4 at N = 256, whose natural
+    // allocation sits at the 128-VGPR boundary (the dense-worker branch would otherwise push it to 3 waves)
+    static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN == 8) ? 4 : 1;
     __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
     __device__ static __forceinline__ int pad(int p) { return p + (p >> LOGE); }
     // physical index of lane-local element u in the grouping of pass K
@@ -191,7 +199,7 @@ __device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cons
 }
 
 template <typename T, int LOGN, int LOGE, bool COMP, int LO>
-__global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegArgs RA) {
+__global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW)) void rows_reg_kernel(RegArgs RA) {
     using C = Cfg<T, LOGN, LOGE>;
     constexpr int E = C::E;
     constexpr int NP = C::NPASS;
@@ -199,6 +207,18 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
     const RowsArgs& A = RA.r;
     __shared__ cplx<T> Xs[C::RPB * C::ROW];
     __shared__ int sdeg[C::RPB];
+
+    // ---- dense workers: the leading blocks take the pass's dense tiles on the matrix cores while the row blocks
+    // keep the vector ALUs busy (no barrier on this path)
+    int bid = blockIdx.x;
+    if (A.dn.workers > 0) {
+        if (bid < A.dn.dblk) {
+            const int w = xcd_block(bid, A.dn.dblk) * (C::NT / 64) + (int)(threadIdx.x >> 6);
+            if (w < A.dn.workers) dense::worker<T, 4>(LOGN, A.dn, w, threadIdx.x & 63, A.Y, A.Z);
+            return;
+        }
+        bid -= A.dn.dblk;
+    }
 
     const int tid = threadIdx.x;
     const int rib = tid / C::G;   // row in block
@@ -210,14 +230,14 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT)) void rows_reg_kernel(RegA
     bool valid;
     if (RA.group_by_k) {
         const int nspb = (RA.SP + C::RPB - 1) / C::RPB;
-        const int kk = blockIdx.x / nspb;
-        const int sg = blockIdx.x % nspb;
+        const int kk = bid / nspb;
+        const int sg = bid % nspb;
         const int k = RA.korder ? RA.korder[kk] : kk;
         const int sp = sg * C::RPB + rib;
         valid = sp < RA.SP;
         row = (long long)(valid ? sp : 0) * A.period + k;
     } else {
-        row = (long long)blockIdx.x * C::RPB + rib;
+        row = (long long)bid * C::RPB + rib;
         valid = row < A.rows;
         if (!valid) row = 0;
     }
@@ -409,12 +429,22 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         nblk = (long long)RA.nk * ((RA.SP + C::RPB - 1) / C::RPB);
     else
         nblk = (RA.r.rows + C::RPB - 1) / C::RPB;
+    RegArgs RB = RA;
+    if (RA.r.dn.n_dense > 0) {
+        // dense workers in leading blocks of C::NT / 64 waves, padded to a multiple of the XCD count
+        RB.r.dn.workers = dense_workers(LOGN, RA.r.dn);
+        const int wpb = C::NT / 64;
+        RB.r.dn.dblk = (int)xcd_grid((RB.r.dn.workers + wpb - 1) / wpb);
+        nblk += RB.r.dn.dblk;
+    } else {
+        RB.r.dn.workers = RB.r.dn.dblk = 0;
+    }
     if (nblk < 1) return OK;
     if (nblk > 0x7fffffffLL) {
         set_error("rows_reg: too many blocks");
         return E_INVALID;
     }
-    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RA);
+    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RB);
     return launch_check("rows_reg_kernel");
 }
 

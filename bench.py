@@ -39,10 +39,19 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
+def dense_work(H, S):
+    """FP64 MFMA FLOPs and HBM bytes of the dense tiles in one row-kernel launch over S sinograms: per dense frequency
+    the 2S rows times the N x N complex pass matrix (8 N^2 flops per row), the matrix read once (complex128) and the
+    rows read and written once (complex64)."""
+    N, nd, SP = H.N, H.eng.n_dense, 2 * S
+    return 8.0 * nd * SP * N * N, nd * N * N * 16.0 + 2.0 * nd * SP * N * 8
+
+
 def rows_kernel_work(H, S, ops):
-    """Algorithmic FLOPs and HBM bytes of one row-kernel launch over S sinograms.  The kernel takes the rows of the
-    non-dense frequencies (both packed pairs of every sinogram); per row: 8N flops per B / BT level and
-    (d - 1)(16 N U + 18 N) + 2N per Chebyshev (P) op over U levels; it reads and writes each row once (complex64)."""
+    """Algorithmic FLOPs (vector ALU) and HBM bytes of one row-kernel launch over S sinograms.  The kernel takes the
+    rows of the non-dense frequencies (both packed pairs of every sinogram); per row: 8N flops per B / BT level and
+    (d - 1)(16 N U + 18 N) + 2N per Chebyshev (P) op over U levels; it reads and writes each row once (complex64).
+    The bytes include the dense tiles of the same launch (dense_work); their FP64 MFMA flops are counted apart."""
     N, Lr = H.N, H.L
     dense = np.zeros(Lr, dtype=bool)
     dense[H.eng.dense_k] = True
@@ -54,7 +63,7 @@ def rows_kernel_work(H, S, ops):
         if o == "P":
             U = hi - lo + 1
             f += rows_per_k * ((np.maximum(d - 1, 0)) * (16 * N * U + 18 * N) + 2 * N).sum()
-    nbytes = 2.0 * rows_per_k * (~dense).sum() * N * 8
+    nbytes = 2.0 * rows_per_k * (~dense).sum() * N * 8 + dense_work(H, S)[1]
     return f, nbytes
 
 
@@ -347,13 +356,16 @@ def main():
     H32.operator_device(d_x32, d_tmp, B, st)  # leaves the rows of a real G_h pass in the plan
     rows_ms, _ = time_rows_kernel(L, H32, B, st)
     fl, nb = rows_kernel_work(H32, B, ops)
+    dfl = dense_work(H32, B)[0]
     achieved = fl / (rows_ms * 1e-3)
     roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK["fp32"], "traffic": pmc_traffic(),
             "frac_nonpacked": achieved / PEAK["fp32_nonpacked"],
-            "kernel": f"rows_reg_kernel<float, {n}, e, comp> (G_h pass)", "kernel_ms": rows_ms,
-            "flops_per_launch": fl, "alg_bytes_per_launch": nb, "units_per_launch": f"{2 * B} rows x "
-            f"{H32.L - H32.eng.n_dense} non-dense frequencies"}
+            "kernel": f"rows_reg_kernel<float, {n}, e, comp> (G_h pass, with the dense MFMA tiles)",
+            "kernel_ms": rows_ms, "flops_per_launch": fl, "alg_bytes_per_launch": nb,
+            "units_per_launch": f"{2 * B} rows x {H32.L - H32.eng.n_dense} non-dense frequencies (+ {2 * B} rows x "
+            f"{H32.eng.n_dense} dense frequencies as FP64 MFMA tiles)",
+            "dense_fp64_mfma_flops_per_launch": dfl, "dense_fp64_mfma_tflops": dfl / (rows_ms * 1e-3) / 1e12}
     log(f"row kernel {rows_ms * 1e3:.1f} us: {achieved / 1e12:.1f} TF/s")
 
     mixed = None
