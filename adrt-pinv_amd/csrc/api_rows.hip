@@ -102,6 +102,7 @@ int cheb_rows_impl(int n, int K, const void* z, const int* deg, const double* a,
     ADRT_TRY(ensure(2, sizeof(int) * (size_t)K));
     ADRT_TRY(ensure(3, sizeof(double) * (size_t)K));
     ADRT_TRY(ensure(4, sizeof(double) * (size_t)K));
+    ADRT_TRY(ensure(5, cheb_table_bytes(K, deg)));
     ADRT_HIP(hipMemcpyAsync(s.buf[0], z, sizeof(c128) * K, hipMemcpyHostToDevice, s.stream));
     ADRT_HIP(hipMemcpyAsync(s.buf[1], Y, rb, hipMemcpyHostToDevice, s.stream));
     ADRT_HIP(hipMemcpyAsync(s.buf[2], deg, sizeof(int) * K, hipMemcpyHostToDevice, s.stream));
@@ -120,6 +121,8 @@ int cheb_rows_impl(int n, int K, const void* z, const int* deg, const double* a,
     A.a = (const double*)s.buf[3];
     A.c = (const double*)s.buf[4];
     A.tau = tau;
+    ADRT_TRY(cheb_table_fill(K, deg, (const int*)s.buf[2], (const double*)s.buf[3], (const double*)s.buf[4], s.buf[5],
+                             s.stream, &A.wcoef, &A.wcoff));
     A.nops = 1;
     A.ops[0] = RowOp{OP_P, m_lo, m_hi};
     ADRT_TRY(launch_rows(A, PREC_F64, s.stream));
@@ -202,6 +205,9 @@ struct RowsPlan {
     int* dslot = nullptr;
     c128* dFT = nullptr;
     int n_dense = 0;
+    void* wtab = nullptr;  // Chebyshev step weights (cheb_table_fill)
+    const c128* wcoef = nullptr;
+    const int* wcoff = nullptr;
 };
 
 }  // namespace adrt
@@ -381,6 +387,13 @@ int adrt_rows_plan_create(adrt_rows_plan** out, int n, int L, const adrt_c128* z
         set_error("rows_plan_create: upload failed");
         return fail(E_HIP);
     }
+    if ((st = adrt_malloc(&p->wtab, cheb_table_bytes(L, deg))) != OK ||
+        (st = cheb_table_fill(L, deg, p->deg, p->a, p->c, p->wtab, nullptr, &p->wcoef, &p->wcoff)) != OK)
+        return fail(st);
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        set_error("rows_plan_create: Chebyshev weight table failed");
+        return fail(E_HIP);
+    }
     if (n_dense > 0) {
         std::vector<int> slot(L, -1);
         for (int i = 0; i < n_dense; ++i) {
@@ -417,6 +430,7 @@ int adrt_rows_plan_destroy(adrt_rows_plan* plan) {
     (void)hipFree(p->c);
     (void)hipFree(p->dslot);
     (void)hipFree(p->dFT);
+    (void)hipFree(p->wtab);
     delete p;
     return OK;
 }
@@ -445,6 +459,8 @@ int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows
     A.tau = p->tau;
     A.dslot = p->dslot;
     A.dFT = p->dFT;
+    A.wcoef = p->wcoef;
+    A.wcoff = p->wcoff;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = RowOp{ops[3 * i], ops[3 * i + 1], ops[3 * i + 2]};
     A.comp = p->comp;

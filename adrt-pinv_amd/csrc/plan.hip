@@ -21,6 +21,9 @@ struct Plan {
     double* c = nullptr;
     int* dslot = nullptr;
     c128* dFT = nullptr;
+    void* wtab = nullptr;          // Chebyshev step weights of every frequency (cheb_table_fill)
+    const c128* wcoef = nullptr;   // (inside wtab)
+    const int* wcoff = nullptr;
     int* korder = nullptr;  // frequencies by descending degree (block order of the row kernel)
     int* korder_nd = nullptr;  // the same without the dense frequencies
     int* dense_k = nullptr;    // dense frequencies (GEMM slots)
@@ -54,7 +57,7 @@ size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(
 void free_plan(Plan* p) {
     if (!p) return;
     if (p->borrowed) p->dFT = p->WopT = p->WbackT = nullptr;
-    void* bufs[] = {p->z, p->deg, p->a, p->c, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->wtab, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -79,6 +82,8 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
     A.tau = p->tau;
     A.dslot = p->dslot;
     A.dFT = p->dFT;
+    A.wcoef = p->wcoef;
+    A.wcoff = p->wcoff;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = ops[i];
     A.group_by_k = 1;
@@ -179,6 +184,12 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
     const size_t N = (size_t)1 << d->n, L = 2 * N, S = d->max_batch > 0 ? d->max_batch : 1;
     size_t b = 0;
     b += L * (sizeof(c128) + sizeof(int) + 2 * sizeof(double) + sizeof(int));
+    if (d->deg) {
+        b += cheb_table_bytes((int)L, d->deg);  // Chebyshev weight table
+    } else {
+        const std::vector<int> zdeg(L, 0);
+        b += cheb_table_bytes((int)L, zdeg.data());
+    }
     if (d->dense_on_device != 2)
         b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
     b += twiddle_bytes(d->n, PREC_F64) + twiddle_bytes(d->n, PREC_F32);
@@ -245,6 +256,13 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     if ((st = alloc((void**)&p->deg, sizeof(int) * L)) || (st = up(p->deg, degs, sizeof(int) * L))) return fail(st);
     if ((st = alloc((void**)&p->a, sizeof(double) * L)) || (st = up(p->a, as, sizeof(double) * L))) return fail(st);
     if ((st = alloc((void**)&p->c, sizeof(double) * L)) || (st = up(p->c, cs, sizeof(double) * L))) return fail(st);
+    if ((st = alloc(&p->wtab, cheb_table_bytes(L, degs))) ||
+        (st = cheb_table_fill(L, degs, p->deg, p->a, p->c, p->wtab, nullptr, &p->wcoef, &p->wcoff)))
+        return fail(st);
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        set_error("plan_create: Chebyshev weight table failed");
+        return fail(E_HIP);
+    }
     if (d->n_dense > 0) {
         std::vector<int> slot(L, -1);
         for (int i = 0; i < d->n_dense; ++i) {

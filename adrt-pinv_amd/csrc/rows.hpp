@@ -38,6 +38,10 @@ struct RowsArgs {
     int skip_dense = 0;  // rows of dense frequencies are left untouched (a dense GEMM handles them)
     int loge = 0;  // register kernel elements per lane (log2); 0 = default for (n, precision), -1 = generic kernel
     int comp = 0;  // FP32 only: compensated (TwoSum) accumulation of the Chebyshev iterate
+    // Chebyshev step weights (w_j a, w_j - 1) of parameter row k at wcoef[wcoff[k] + j], 1 <= j < deg[k]
+    // (cheb_table_fill); the register kernel takes op lists with P ops only when they are given
+    const c128* wcoef = nullptr;
+    const int* wcoff = nullptr;
     // dense rows of the pass for the register kernel to compute as MFMA tiles beside the rows (n_dense = 0: none);
     // requires skip_dense and Z distinct from Y
     DenseArgs dn{};
@@ -54,6 +58,14 @@ int launch_rows_reg(const RowsArgs& args, int prec, int loge, hipStream_t stream
 // others), never more than the pass has tiles.
 int dense_workers(int n, const DenseArgs& dn);
 int default_loge(int n, int prec);
+// Table of the Chebyshev step weights of the register kernel's P op for K parameter rows, in one device allocation
+// of cheb_table_bytes(K, deg) bytes: K offsets, then wcoef[off[k] + j] = (w_j a_k, w_j - 1) for 1 <= j < deg[k]
+// (w_1 = 1 / (1 - c^2 / 2), w_j = 1 / (1 - c^2 w_(j-1) / 4), the reference's cpp/oblique.cpp:86-98), max(deg[k], 1)
+// entries per row.  deg_host is the host copy of d_deg; the weights are computed on the stream, the offsets are
+// copied before return.
+size_t cheb_table_bytes(int K, const int* deg_host);
+int cheb_table_fill(int K, const int* deg_host, const int* d_deg, const double* d_a, const double* d_c, void* d_table,
+                    hipStream_t st, const c128** wcoef, const int** wcoff);
 int rows_config_get(int n, int prec);
 int rows_config_set(int n, int prec, int loge);
 

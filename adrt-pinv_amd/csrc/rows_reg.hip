@@ -2,7 +2,9 @@
 // instantiation units rows_reg_{d,s,c}{1,2}.hip.
 #include "rows_reg.hpp"
 
+#include <algorithm>
 #include <atomic>
+#include <vector>
 
 namespace adrt {
 
@@ -63,10 +65,62 @@ int dense_workers(int n, const DenseArgs& dn) {
     return (int)(T < sm ? T : sm);
 }
 
+namespace {
+
+// One thread per frequency; the recurrence is the one the register kernel evaluates without a table.
+__global__ __launch_bounds__(256) void cheb_coef_kernel(int L, const int* deg, const double* a, const double* c,
+                                                         const int* off, c128* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L) return;
+    const double ak = a[k], ck = c[k];
+    c128* o = out + off[k];
+    o[0] = mk(0.0, 0.0);
+    double om = 1.0;
+    for (int it = 1; it < deg[k]; ++it) {
+        const double omj = (it == 1) ? 1.0 / (1.0 - ck * ck / 2.0) : 1.0 / (1.0 - ck * ck * om / 4.0);
+        o[it] = mk(omj * ak, omj - 1.0);
+        om = omj;
+    }
+}
+
+}  // namespace
+
+namespace {
+size_t cheb_table_head(int K) { return ((sizeof(int) * (size_t)K + 255) / 256) * 256; }
+}  // namespace
+
+size_t cheb_table_bytes(int K, const int* deg_host) {
+    size_t e = 0;
+    for (int k = 0; k < K; ++k) e += (size_t)std::max(deg_host[k], 1);
+    return cheb_table_head(K) + sizeof(c128) * e;
+}
+
+int cheb_table_fill(int K, const int* deg_host, const int* d_deg, const double* d_a, const double* d_c, void* d_table,
+                    hipStream_t st, const c128** wcoef, const int** wcoff) {
+    ADRT_REQUIRE(K >= 1 && deg_host && d_deg && d_a && d_c && d_table, "cheb_table_fill: bad argument");
+    std::vector<int> off(K);
+    long long tot = 0;
+    for (int k = 0; k < K; ++k) {
+        off[k] = (int)tot;
+        tot += std::max(deg_host[k], 1);
+    }
+    ADRT_REQUIRE(tot <= 0x7fffffffLL, "cheb_table_fill: Chebyshev degrees too large");
+    int* d_off = (int*)d_table;
+    c128* d_w = (c128*)((char*)d_table + cheb_table_head(K));
+    ADRT_HIP(hipMemcpy(d_off, off.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(cheb_coef_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, d_deg, d_a, d_c, d_off, d_w);
+    ADRT_TRY(launch_check("cheb_coef_kernel"));
+    *wcoef = d_w;
+    *wcoff = d_off;
+    return OK;
+}
+
 int launch_rows_reg(const RowsArgs& A, int prec, int loge, hipStream_t s) {
     RegArgs RA{};
     RA.r = A;
     if (!decode_ops(A.n, A.ops, A.nops, RA)) return E_INVALID;
+    for (int o = 0; o < RA.nmops; ++o)
+        if (RA.mop[o] == MOP_P && !(A.wcoef && A.wcoff)) return E_INVALID;  // P needs the weight table
     if (A.in_stride != A.N || A.out_stride != A.N) return E_INVALID;
     RA.group_by_k = A.group_by_k;
     RA.SP = A.SP;

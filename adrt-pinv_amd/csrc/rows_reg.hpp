@@ -340,8 +340,10 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
             constexpr int EL = COMP ? E : 1;
             constexpr int EP = COMP ? 1 : E;
             cplx<T> y[E], zi[E], zp[EP], zl[EL], dl[EL];
-            const double ak = valid ? A.a[k] : 0.0, ck = valid ? A.c[k] : 0.0;
+            const double ak = valid ? A.a[k] : 0.0;
             const T aT = (T)ak, tau = (T)A.tau;
+            // the step weights come from a table (no double division in the loop)
+            const c128* wc = A.wcoef + A.wcoff[k];
 #pragma unroll
             for (int u = 0; u < E; ++u) {
                 y[u] = v[u];
@@ -353,15 +355,14 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
                     zp[u] = mk((T)0, (T)0);
                 }
             }
-            double om = 1.0;
             for (int it = 1; it < dmax; ++it) {
-                const double omj = (it == 1) ? 1.0 / (1.0 - ck * ck / 2.0) : 1.0 / (1.0 - ck * ck * om / 4.0);
+                const c128 wq = wc[it < d ? it : 0];  // loaded before the Gram: its latency hides behind it
 #pragma unroll
                 for (int u = 0; u < E; ++u) v[u] = zi[u];
                 gram_rec<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, lane);
                 if (it < d) {
                     // z <- z + (w a) r + (w - 1)(z - z_prev), r = y - M z - tau z
-                    const T wa = (T)(omj * ak), w0 = (T)(omj - 1.0);
+                    const T wa = (T)wq.x, w0 = (T)wq.y;
 #pragma unroll
                     for (int u = 0; u < E; ++u) {
                         if constexpr (COMP) {
@@ -378,7 +379,6 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
                             zi[u] = zn;
                         }
                     }
-                    om = omj;
                 }
             }
             if (anyd) {
