@@ -6,6 +6,11 @@ register passes with one LDS exchange between passes.  Larger E means fewer pass
 parallelism but more registers (lower occupancy).  The tuner times the plan's own row pass (events on the plan's
 stream) for each candidate (e = 2, 3, 4 and the generic LDS kernel) on the rows of a real pass, keeps the fastest and
 records the choice in the library (adrt_rows_config_set) and in a JSON file next to the plan cache.
+
+With the mapping fixed it then places the dense low-frequency GEMM of the pass (plans with dense rows): as MFMA waves
+inside the row kernel's launch, at most 1, 2 or 4 per SIMD, or as a launch of its own after the rows
+(adrt_dense_config_set).  Few dense waves leave the vector ALUs to the rows; more hide the GEMM's load latency better
+when the dense work is large (N >= 1024, where a third of the frequencies can be dense).
 """
 
 import json
@@ -15,6 +20,17 @@ from . import lib as L
 from . import plancache
 
 CANDIDATES = (-1, 2, 3, 4)
+DENSE_CANDIDATES = (-1, 1, 2, 4)
+
+
+class TuneResult(tuple):
+    """(loge, {loge: ms}); the dense placement chosen after it is in .dense, its timings in .dense_times."""
+
+    def __new__(cls, loge, times, dense=0, dense_times=None):
+        r = super().__new__(cls, (loge, times))
+        r.dense = dense
+        r.dense_times = dense_times or {}
+        return r
 
 
 def _table_path():
@@ -27,7 +43,11 @@ def load_table():
     if p and os.path.exists(p):
         try:
             with open(p) as f:
-                return {tuple(map(int, k.split(","))): int(v) for k, v in json.load(f).items()}
+                out = {}
+                for k, v in json.load(f).items():
+                    loge, dense = (int(v), 0) if isinstance(v, int) else (int(v[0]), int(v[1]))
+                    out[tuple(map(int, k.split(",")))] = (loge, dense)
+                return out
         except (OSError, ValueError):
             return {}
     return {}
@@ -39,20 +59,31 @@ def save_table(table):
         return
     tmp = p + f".{os.getpid()}.tmp"
     with open(tmp, "w") as f:
-        json.dump({f"{n},{prec}": v for (n, prec), v in table.items()}, f)
+        json.dump({f"{n},{prec}": [int(v[0]), int(v[1])] for (n, prec), v in table.items()}, f)
     os.replace(tmp, p)
 
 
 def apply_table(table=None):
-    """Push a tuning table {(n, precision_code): loge} into the library."""
+    """Push a tuning table {(n, precision_code): (loge, dense)} into the library."""
     lib = L.load()
-    for (n, prec), v in (table if table is not None else load_table()).items():
-        lib.adrt_rows_config_set(n, prec, v)
+    for (n, prec), (loge, dense) in (table if table is not None else load_table()).items():
+        lib.adrt_rows_config_set(n, prec, loge)
+        lib.adrt_dense_config_set(n, prec, dense)
 
 
-def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, persist=True):
+def _time_pass(lib, plan, S, st, e0, e1, reps):
+    L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")  # warm-up
+    e0.record(st)
+    for _ in range(reps):
+        L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")
+    e1.record(st)
+    return e0.elapsed_ms(e1) / reps
+
+
+def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=DENSE_CANDIDATES, persist=True):
     """Time the row pass of ``plan`` (a FastLpHip / FastHybridHip with S <= max_batch sinograms) for every candidate
-    mapping; install and return (best_loge, {loge: ms})."""
+    mapping, then (plans with dense rows) every dense placement at the chosen mapping; install both and return
+    TuneResult(best_loge, {loge: ms}) with .dense and .dense_times."""
     lib = L.load()
     prec = L.CONFIG_PRECISION[plan.precision]
     n = plan.n
@@ -64,20 +95,24 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, persist=True):
     e0, e1 = L.Event(), L.Event()
     times = {}
     old = lib.adrt_rows_config_get(n, prec)
+    old_dense = lib.adrt_dense_config_get(n, prec)
+    L.check(lib.adrt_dense_config_set(n, prec, 0), "dense_config_set")
     for c in candidates:
         if c > 0 and (1 << c) > plan.N:
             continue
         L.check(lib.adrt_rows_config_set(n, prec, c), "rows_config_set")
-        L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")  # warm-up
-        e0.record(st)
-        for _ in range(reps):
-            L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")
-        e1.record(st)
-        times[c] = e0.elapsed_ms(e1) / reps
+        times[c] = _time_pass(lib, plan, S, st, e0, e1, reps)
     best = min(times, key=times.get) if times else old
     lib.adrt_rows_config_set(n, prec, best)
+    dense, dense_times = old_dense, {}
+    if getattr(plan.eng, "n_dense", 0) > 0 and best != -1:  # the generic kernel has no dense waves
+        for m in dense_candidates:
+            L.check(lib.adrt_dense_config_set(n, prec, m), "dense_config_set")
+            dense_times[m] = _time_pass(lib, plan, S, st, e0, e1, reps)
+        dense = min(dense_times, key=dense_times.get)
+    lib.adrt_dense_config_set(n, prec, dense)
     if persist:
         table = load_table()
-        table[(n, prec)] = best
+        table[(n, prec)] = (best, dense)
         save_table(table)
-    return best, times
+    return TuneResult(best, times, dense, dense_times)

@@ -337,6 +337,8 @@ int omp_max_threads(void) {
 
 int adrt_rows_config_set(int n, int precision, int loge) { return rows_config_set(n, precision, loge); }
 int adrt_rows_config_get(int n, int precision) { return rows_config_get(n, precision); }
+int adrt_dense_config_set(int n, int precision, int mode) { return dense_config_set(n, precision, mode); }
+int adrt_dense_config_get(int n, int precision) { return dense_config_get(n, precision); }
 
 int adrt_bfly_rows(int n, int K, const adrt_c128* z, adrt_c128* X, int m_lo, int m_hi, int transpose) {
     return bfly_rows_impl(n, K, z, X, m_lo, m_hi, transpose);

@@ -298,6 +298,20 @@ int rows_config_set(int n, int prec, int loge) {
     return OK;
 }
 
+static int g_dense_cfg[2][12];
+
+int dense_config_get(int n, int prec) {
+    if (n < 0 || n > 11 || prec < 0 || prec > 1) return 0;
+    return g_dense_cfg[prec][n];
+}
+
+int dense_config_set(int n, int prec, int mode) {
+    ADRT_REQUIRE(n >= 0 && n <= 11 && (prec == 0 || prec == 1), "dense_config_set: bad n or precision");
+    ADRT_REQUIRE(mode >= -1 && mode <= 8, "dense_config_set: mode must be -1 .. 8");
+    g_dense_cfg[prec][n] = mode;
+    return OK;
+}
+
 int check_ops(int n, const RowOp* ops, int nops) {
     ADRT_REQUIRE(nops >= 0 && nops <= MAX_OPS, "op list longer than 8");
     for (int i = 0; i < nops; ++i) {
@@ -319,9 +333,12 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
         const int cfg = rows_config_get(A.n, prec);
         const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));
         if (loge > 0) {
-            const int st = launch_rows_reg(A, prec, loge, s);
+            const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, prec) >= 0;
+            RowsArgs R = A;
+            if (!fuse) R.dn = DenseArgs{};
+            const int st = launch_rows_reg(R, prec, loge, s);
             if (st == OK) {
-                if (dense_done) *dense_done = A.dn.n_dense > 0;
+                if (dense_done) *dense_done = fuse;
                 return OK;
             }
             if (st != E_INVALID) return st;

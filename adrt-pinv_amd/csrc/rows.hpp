@@ -54,9 +54,9 @@ int launch_rows(const RowsArgs& args, int prec, hipStream_t stream, int* dense_d
 
 // Register-resident kernel (rows_reg.hip); E_INVALID if the op list or size needs the generic kernel.
 int launch_rows_reg(const RowsArgs& args, int prec, int loge, hipStream_t stream);
-// Dense waves the register kernel adds for args.dn: one per SIMD of the device at most (the row waves keep the
-// others), never more than the pass has tiles.
-int dense_workers(int n, const DenseArgs& dn);
+// Dense waves the register kernel adds for args.dn: at most dense_config_get(n, prec) (default 1) per SIMD of the
+// device, the row waves keeping the others, and never more than the pass has tiles.
+int dense_workers(int n, int prec, const DenseArgs& dn);
 int default_loge(int n, int prec);
 // Table of the Chebyshev step weights of the register kernel's P op for K parameter rows, in one device allocation
 // of cheb_table_bytes(K, deg) bytes: K offsets, then wcoef[off[k] + j] = (w_j a_k, w_j - 1) for 1 <= j < deg[k]
@@ -68,6 +68,9 @@ int cheb_table_fill(int K, const int* deg_host, const int* d_deg, const double* 
                     hipStream_t st, const c128** wcoef, const int** wcoff);
 int rows_config_get(int n, int prec);
 int rows_config_set(int n, int prec, int loge);
+// dense GEMM placement per (n, prec): 0 default, -1 own launch, k > 0 at most k dense waves per SIMD in the row launch
+int dense_config_get(int n, int prec);
+int dense_config_set(int n, int prec, int mode);
 
 // Dense rows as a GEMM with the per-frequency pass matrices W^T (dense.hip).
 int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,

@@ -50,7 +50,7 @@ int default_loge(int n, int prec) {
     return 2;
 }
 
-int dense_workers(int n, const DenseArgs& dn) {
+int dense_workers(int n, int prec, const DenseArgs& dn) {
     static std::atomic<int> simds[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -61,6 +61,8 @@ int dense_workers(int n, const DenseArgs& dn) {
         sm = 4 * cus;
         simds[dev].store(sm, std::memory_order_relaxed);
     }
+    const int per = dense_config_get(n, prec);
+    if (per > 1) sm *= per;
     const long long T = dense::tiles(n, dn);
     return (int)(T < sm ? T : sm);
 }

@@ -74,9 +74,12 @@ struct Cfg {
     static constexpr int RPB = NT / G;
     static constexpr int NPASS = (LOGN + LOGE - 1) / LOGE;
     static constexpr int ROW = N + (N >> LOGE);  // padded row length in LDS (also holds the z^t table, N/2 + 1)
-    // waves per SIMD the register allocation must allow: four for float rows of E = 4 at N = 256, whose natural
-    // allocation sits at the 128-VGPR boundary (the dense-worker branch would otherwise push it to 3 waves)
-    static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN == 8) ? 4 : 1;
+    // Waves per SIMD the register allocation must allow (measured on MI355X): float rows with E = 4 from N = 128 on
+    // run faster at 4 waves even with a few spilled registers than at their natural 3, and E = 8 up to N = 512 at 3
+    // rather than 2; for N >= 1024 with E = 8 the spills cost more than the extra wave.
+    static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN >= 7)   ? 4
+                                : (sizeof(T) == 4 && LOGE == 3 && LOGN <= 9) ? 3
+                                                                             : 1;
     __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
     __device__ static __forceinline__ int pad(int p) { return p + (p >> LOGE); }
     // physical index of lane-local element u in the grouping of pass K
@@ -432,7 +435,7 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
     RegArgs RB = RA;
     if (RA.r.dn.n_dense > 0) {
         // dense workers in leading blocks of C::NT / 64 waves, padded to a multiple of the XCD count
-        RB.r.dn.workers = dense_workers(LOGN, RA.r.dn);
+        RB.r.dn.workers = dense_workers(LOGN, sizeof(T) == 8 ? PREC_F64 : PREC_F32, RA.r.dn);
         const int wpb = C::NT / 64;
         RB.r.dn.dblk = (int)xcd_grid((RB.r.dn.workers + wpb - 1) / wpb);
         nblk += RB.r.dn.dblk;

@@ -49,6 +49,8 @@ SIGNATURES = {
     "adrt_event_query": (I, [P]),
     "adrt_rows_config_set": (I, [I, I, I]),
     "adrt_rows_config_get": (I, [I, I]),
+    "adrt_dense_config_set": (I, [I, I, I]),
+    "adrt_dense_config_get": (I, [I, I]),
     "adrt_rows_plan_create": (I, [ctypes.POINTER(P), I, I, P, P, P, P, D, I, P, P, I]),
     "adrt_rows_plan_destroy": (I, [P]),
     "adrt_rows_run": (I, [P, P, P, LL, P, I, P]),

@@ -151,9 +151,10 @@ def mixed_batch_leg(count=512, dense_cap=16):
     ms = e0.elapsed_ms(e1) / reps
     counts = {int(1 << n): ns.count(n) for n in sorted(set(ns))}
     tuned = {int(1 << n): int(v[0]) for n, v in mb.tuning.items()}
+    dense = {int(1 << n): int(v.dense) for n, v in mb.tuning.items()}
     mb.close()
     return {"sinograms": count, "sizes": counts, "ms_per_launch": ms, "inverses_per_s": count * 1000.0 / ms,
-            "setup_s": setup_s, "row_mapping_loge": tuned, "dense_cap": dense_cap}
+            "setup_s": setup_s, "row_mapping_loge": tuned, "dense_placement": dense, "dense_cap": dense_cap}
 
 
 SERVE_BATCH = {7: 16, 8: 16, 9: 8, 10: 8}  # sinograms per launch by log2 N
@@ -299,8 +300,10 @@ def main():
                           refine=a.refine)
     log(f"setup {setup_s:.1f}s: D={H64.outer_deg} sum(deg)/N={H64.deg.sum() / N:.1f} dense rows={H64.eng.n_dense}")
     # the row kernel's mapping for this (N, FP32) chosen by the project's autotuner on the plan's own row pass
-    loge, tune_ms = pkg("autotune").tune(H32, B, persist=False)
-    log(f"autotuned row mapping: loge={loge} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tune_ms.items())})")
+    tuned = pkg("autotune").tune(H32, B, persist=False)
+    loge, tune_ms = tuned
+    log(f"autotuned row mapping: loge={loge} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tune_ms.items())}); "
+        f"dense placement {tuned.dense} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.dense_times.items())})")
 
     # inputs: ellipse phantoms (seeds 1000+i) through the GPU forward ADRT, plus 1% noise (seeds 2000+i)
     Bn = make_sinos(n, B, 1000, 2000)
@@ -413,7 +416,7 @@ def main():
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
                    "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
                    "sum_deg_over_N": float(H64.deg.sum() / N), "n_dense": H64.eng.n_dense,
-                   "row_mapping_loge": int(loge)},
+                   "row_mapping_loge": int(loge), "dense_placement": int(tuned.dense)},
         "plain_fp64_inverses_per_s": plain_ips,
         "speedup_vs_plain_fp64": ips / plain_ips,
         "relerr_fp32_vs_fp64_max": float(rel.max()),

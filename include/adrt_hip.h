@@ -93,6 +93,13 @@ int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows
 int adrt_rows_config_set(int n, int precision, int loge);
 int adrt_rows_config_get(int n, int precision);
 
+/* Dense-row GEMM placement per (n, precision) for passes whose dense rows take the MFMA path: 0 = built-in default
+ * (one dense wave per SIMD inside the register row kernel's launch), -1 = a launch of its own after the rows,
+ * 1..8 = at most that many dense waves per SIMD inside the row kernel's launch.  Chosen by the autotuner.  (No
+ * counterpart in the reference, whose dense rows run inside its row loop.) */
+int adrt_dense_config_set(int n, int precision, int mode);
+int adrt_dense_config_get(int n, int precision);
+
 /* Inverse plans: fastops.FastLp / FastHybrid (fastops.py:261-338) with the setup scalars given.
  * All image / sinogram buffers are float64 on the device:
  *   b: (S, 4, 2N-1, N) sinograms, x / y: (S, N, N) images. */

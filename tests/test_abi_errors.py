@@ -74,3 +74,18 @@ def test_memory_and_plan_calls_reject(L):
     assert lib.adrt_plan_apply(None, None, None, 1, None) == -1
     h = L.P()
     assert lib.adrt_rows_plan_create(ctypes.byref(h), 3, 16, None, None, None, None, 1.0, 0, None, None, 0) == -1
+
+
+def test_dense_config_validates(L):
+    lib = L.load()
+    assert lib.adrt_dense_config_set(12, 0, 1) == -1 and "n" in err(L)
+    assert lib.adrt_dense_config_set(8, 2, 1) == -1 and "precision" in err(L)
+    assert lib.adrt_dense_config_set(8, 1, 9) == -1 and "mode" in err(L)
+    assert lib.adrt_dense_config_set(8, 1, -2) == -1
+    old = lib.adrt_dense_config_get(8, 1)
+    try:
+        for m in (-1, 0, 1, 2, 4, 8):
+            assert lib.adrt_dense_config_set(8, 1, m) == 0
+            assert lib.adrt_dense_config_get(8, 1) == m
+    finally:
+        lib.adrt_dense_config_set(8, 1, old)

@@ -180,6 +180,34 @@ def test_fp32_batch_headline_config(gpu):
     assert np.array_equal(x32[2], H32.apply(B[2]))
 
 
+@pytest.mark.parametrize("prec,n", [("fp32", 8), ("fp64", 8), ("fp32", 9)])
+def test_dense_placement_same_bits(gpu, prec, n):
+    """The dense low-frequency GEMM gives the same bits whichever launch computes it (a launch of its own, or 1, 2, 4
+    MFMA waves per SIMD inside the row kernel's launch): every placement runs the same tile arithmetic in the same
+    order.  Batch 3 puts FP64 passes on the MFMA path too (6 rows > 4)."""
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    lib = gpu.load()
+    S, cap = 3, 16
+    N = 1 << n
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(300 + i, N)), 0.01, 400 + i) for i in range(S)])
+    H = I.FastHybridHip(n, dense_cap=cap, precision=prec, max_batch=S, refine=1)
+    assert H.eng.n_dense > 0
+    p = gpu.CONFIG_PRECISION[prec]
+    old = lib.adrt_dense_config_get(n, p)
+    try:
+        out = {}
+        for m in (-1, 1, 2, 4):
+            gpu.check(lib.adrt_dense_config_set(n, p, m), "dense_config_set")
+            out[m] = H.apply_batch(B)
+        for m in (1, 2, 4):
+            assert np.array_equal(out[m], out[-1]), m
+    finally:
+        lib.adrt_dense_config_set(n, p, old)
+        H.close()
+
+
 def test_plan_batch_limits(gpu):
     """A batch above the plan's max_batch is an error, not an overrun; an empty batch is a no-op."""
     L = gpu
