@@ -140,8 +140,8 @@ __global__ __launch_bounds__(FFT_THREADS) void ifft_unpack_kernel(int n, int C, 
 }
 
 // out[a][c] = (S0[N-1-c][a] + S1[N-1-a][c]) + (S2[a][c] + S3[N-1-c][N-1-a]), optionally y - out.
-template <typename Tg>
-__global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const double* y, double* out) {
+template <typename Tg, typename To>
+__global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const double* y, To* out) {
     __shared__ double t0[32][33], t3[32][33];
     const int N = 1 << n;
     const int s = blockIdx.z;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const
         const double v2 = (double)S2[(size_t)a * N + c];
         const double v = (t0[tx][r] + v1) + (v2 + t3[tx][r]);
         const size_t o = ((size_t)s * N + a) * N + c;
-        out[o] = y ? y[o] - v : v;
+        out[o] = (To)(y ? y[o] - v : v);
     }
 }
 
@@ -229,17 +229,24 @@ int launch_ifft_unpack(int n, const void* Z, void* Sq, int S, int prec, const vo
     return ifft_impl<float, float>(n, Z, Sq, S, tw, st);
 }
 
-int launch_combine(int n, const void* Sq, int prec, const double* y, double* out, int S, hipStream_t st) {
+int launch_combine(int n, const void* Sq, int prec, const double* y, void* out, int S, hipStream_t st, bool out_f32) {
     ADRT_REQUIRE(n >= 1 && n <= 11, "combine: need 1 <= n <= 11");
     if (S <= 0) return OK;
     const int N = 1 << n;
     const int T = N < 32 ? 1 : N / 32;
-    if (prec == PREC_F64)
-        hipLaunchKernelGGL(combine_kernel<double>, dim3(T, T, S), dim3(256), 0, st, n, (const double*)Sq, y,
-                           out);
+    const dim3 grid(T, T, S);
+    if (prec == PREC_F64 && !out_f32)
+        hipLaunchKernelGGL((combine_kernel<double, double>), grid, dim3(256), 0, st, n, (const double*)Sq, y,
+                           (double*)out);
+    else if (prec == PREC_F64)
+        hipLaunchKernelGGL((combine_kernel<double, float>), grid, dim3(256), 0, st, n, (const double*)Sq, y,
+                           (float*)out);
+    else if (!out_f32)
+        hipLaunchKernelGGL((combine_kernel<float, double>), grid, dim3(256), 0, st, n, (const float*)Sq, y,
+                           (double*)out);
     else
-        hipLaunchKernelGGL(combine_kernel<float>, dim3(T, T, S), dim3(256), 0, st, n, (const float*)Sq, y,
-                           out);
+        hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(256), 0, st, n, (const float*)Sq, y,
+                           (float*)out);
     return launch_check("combine_kernel");
 }
 

@@ -94,6 +94,7 @@ struct G1Args {
     int n, deg, mode;
     int S;             // images
     const double* y;   // right-hand side (S, N, N)
+    const float* y32;  // the same in float (float iteration only; y is then null)
     const double* x;   // outer iterate (mode 2)
     const double* xp;  // outer previous iterate (mode 2, may be null = 0)
     double a_out, w1, w0;
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     if (lb < 0) return;
     const int s = lb / (nt * nt);
     const int gi0 = ((lb / nt) % nt) * TILE - HALO, gj0 = (lb % nt) * TILE - HALO;
-    const double* ys = A.y + (size_t)s * N * N;
+    const size_t img = (size_t)s * N * N;
     const T a = (T)(2.0 / 20.0);
     for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = Cell4<T>{{(T)0, (T)0, (T)0, (T)0}};
     __syncthreads();
@@ -189,7 +190,8 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             const int gi = gi0 + 2 * bi + (c >> 1), gj = gj0 + 2 * bj + (c & 1);
             const bool in = b < NB && gi >= 0 && gi < N && gj >= 0 && gj < N;
             inm[e] |= (in ? 1u : 0u) << c;
-            yv[e][c] = in ? (T)ys[(size_t)gi * N + gj] : (T)0;
+            const size_t gy = img + (size_t)gi * N + gj;
+            yv[e][c] = in ? (A.y32 ? (T)A.y32[gy] : (T)A.y[gy]) : (T)0;
             xc[e][c] = a * yv[e][c];
             xq[e][c] = (T)0;
             if constexpr (INCR) {
@@ -311,8 +313,10 @@ int launch_g1_t(const G1Args& A, int S, hipStream_t st) {
 }  // namespace
 
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
-                    double w1, double w0, double* out, int S, hipStream_t st, int prec) {
+                    double w1, double w0, double* out, int S, hipStream_t st, int prec, const float* y32) {
     ADRT_REQUIRE(deg >= 1 && deg - 1 <= G1_MAXH, "g1_fused: g1_deg must be 1..17");
+    ADRT_REQUIRE((y != nullptr) != (y32 != nullptr) && (y32 == nullptr || prec == PREC_F32),
+                 "g1_fused: give y, or y32 for the float iteration");
     ADRT_REQUIRE(n >= 1 && n <= 13, "g1_fused: need 1 <= n <= 13");
     if (S <= 0) return OK;
     G1Args A{};
@@ -320,6 +324,7 @@ int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, 
     A.deg = deg;
     A.mode = mode;
     A.y = y;
+    A.y32 = y32;
     A.x = x;
     A.xp = xp;
     A.a_out = a_out;

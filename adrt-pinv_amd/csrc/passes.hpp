@@ -16,8 +16,10 @@ int launch_pack_fft(int n, int mode, const double* src, void* Y, int S, int prec
 int launch_ifft_unpack(int n, const void* Z, void* Sq, int S, int prec, const void* tw, hipStream_t st);
 
 // out[s] = sum_q TqT(q, Sq[s, q]) windowed to N x N (fastspec.py:154-168), summed as (q0 + q1) + (q2 + q3).
-// If y != nullptr the kernel writes y - out instead (the outer residual).
-int launch_combine(int n, const void* Sq, int prec, const double* y, double* out, int S, hipStream_t st);
+// If y != nullptr the kernel writes y - out instead (the outer residual).  out is double, or float with out_f32 (the
+// residual of an FP32 outer step, which the float G_1 iteration rounds to float anyway).
+int launch_combine(int n, const void* Sq, int prec, const double* y, void* out, int S, hipStream_t st,
+                   bool out_f32 = false);
 
 // Two-step register FFT for 6 <= n <= 9 (fft2.hip); mode 0 / 1 as launch_pack_fft, mode 2 as launch_ifft_unpack.
 bool fft2_supported(int n);
@@ -40,9 +42,11 @@ int launch_g1_step(int n, const double* y, const double* x, const double* xp, do
                    double* out, int S, hipStream_t st);
 // g1_inv(y) (deg Chebyshev steps of G_1 on [4, 16]) in one launch, with an epilogue:
 //   mode 0: out = P y;  mode 1: out = a_out * P y;  mode 2: out = x + w1 * (a_out * P y) + w0 * (x - xp) (xp null = 0)
-// prec PREC_F32 runs the inner iteration in float (inputs, outer iterates and the epilogue stay double).
+// prec PREC_F32 runs the inner iteration in float (outer iterates and the epilogue stay double); its right-hand side
+// may then come as float (y32, with y null).
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
-                    double w1, double w0, double* out, int S, hipStream_t st, int prec = PREC_F64);
+                    double w1, double w0, double* out, int S, hipStream_t st, int prec = PREC_F64,
+                    const float* y32 = nullptr);
 // out = (y - Pi y)/4 + Pi y / 12 (fasthybrid.py:101-111)
 int launch_lorr(int n, const double* y, double* out, int S, hipStream_t st);
 

@@ -128,13 +128,13 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
 // One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
 // mode 0: src is an image batch, mode 1: a sinogram batch.  If resid_y != null, out = resid_y - pass(src).
 int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops, int nops, const double* resid_y,
-                double* out, int S, hipStream_t st) {
+                void* out, int S, hipStream_t st, bool out_f32 = false) {
     const int n = p->d.n;
     const void* tw = prec == PREC_F64 ? p->tw : p->tw32;
     ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, tw, st));
     ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, p->Y2, mode == 1 ? p->WbackT : p->WopT));
     ADRT_TRY(launch_ifft_unpack(n, p->Y2, p->Sq, S, prec, tw, st));
-    return launch_combine(n, p->Sq, prec, resid_y, out, S, st);
+    return launch_combine(n, p->Sq, prec, resid_y, out, S, st, out_f32);
 }
 
 // P (the outer preconditioner): g1_inv by fixed-degree Chebyshev on [4, 16], lorr, or the identity.
@@ -502,9 +502,12 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
         const double* xk = X[(k - 1) % 2];
         const double* xp = (k == 1) ? nullptr : X[k % 2];
         const int pk = (refine > 0 && k >= D - refine) ? PREC_F64 : prec;
-        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st));
+        // an FP32 step's residual feeds only the float G_1 iteration, which rounds it to float: store it as float
+        const bool res32 = fused && pk == PREC_F32;
+        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st, res32));
         if (fused) {
-            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, p->res, xk, xp, a, om, om - 1.0, X[k % 2], S, st, pk));
+            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, res32 ? nullptr : p->res, xk, xp, a, om, om - 1.0,
+                                     X[k % 2], S, st, pk, res32 ? reinterpret_cast<const float*>(p->res) : nullptr));
             continue;
         }
         const double* rr = p->res;
