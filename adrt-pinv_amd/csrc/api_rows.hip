@@ -205,6 +205,7 @@ struct RowsPlan {
     int* dslot = nullptr;
     c128* dFT = nullptr;
     int n_dense = 0;
+    void* ztab = nullptr;  // z_k^t per parameter row in the row type (ztab_fill)
     void* wtab = nullptr;  // Chebyshev step weights (cheb_table_fill)
     const c128* wcoef = nullptr;
     const int* wcoff = nullptr;
@@ -390,7 +391,9 @@ int adrt_rows_plan_create(adrt_rows_plan** out, int n, int L, const adrt_c128* z
         return fail(E_HIP);
     }
     if ((st = adrt_malloc(&p->wtab, cheb_table_bytes(L, deg))) != OK ||
-        (st = cheb_table_fill(L, deg, p->deg, p->a, p->c, p->wtab, nullptr, &p->wcoef, &p->wcoff)) != OK)
+        (st = cheb_table_fill(L, deg, p->deg, p->a, p->c, p->wtab, nullptr, &p->wcoef, &p->wcoff)) != OK ||
+        (st = adrt_malloc(&p->ztab, ztab_bytes(n, L, p->prec))) != OK ||
+        (st = ztab_fill(n, L, p->z, p->prec, p->ztab, nullptr)) != OK)
         return fail(st);
     if (hipStreamSynchronize(nullptr) != hipSuccess) {
         set_error("rows_plan_create: Chebyshev weight table failed");
@@ -433,6 +436,7 @@ int adrt_rows_plan_destroy(adrt_rows_plan* plan) {
     (void)hipFree(p->dslot);
     (void)hipFree(p->dFT);
     (void)hipFree(p->wtab);
+    (void)hipFree(p->ztab);
     delete p;
     return OK;
 }
@@ -463,6 +467,7 @@ int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows
     A.dFT = p->dFT;
     A.wcoef = p->wcoef;
     A.wcoff = p->wcoff;
+    A.ztab = p->ztab;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = RowOp{ops[3 * i], ops[3 * i + 1], ops[3 * i + 2]};
     A.comp = p->comp;

@@ -21,6 +21,8 @@ struct Plan {
     double* c = nullptr;
     int* dslot = nullptr;
     c128* dFT = nullptr;
+    void* ztab64 = nullptr;        // z_k^t, t <= N/2, per frequency, complex128 (ztab_fill)
+    void* ztab32 = nullptr;        // the same in complex64 (FP32 plans)
     void* wtab = nullptr;          // Chebyshev step weights of every frequency (cheb_table_fill)
     const c128* wcoef = nullptr;   // (inside wtab)
     const int* wcoff = nullptr;
@@ -57,7 +59,7 @@ size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(
 void free_plan(Plan* p) {
     if (!p) return;
     if (p->borrowed) p->dFT = p->WopT = p->WbackT = nullptr;
-    void* bufs[] = {p->z, p->deg, p->a, p->c, p->wtab, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->wtab, p->ztab64, p->ztab32, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -84,6 +86,7 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
     A.dFT = p->dFT;
     A.wcoef = p->wcoef;
     A.wcoff = p->wcoff;
+    A.ztab = prec == PREC_F64 ? p->ztab64 : p->ztab32;
     A.nops = nops;
     for (int i = 0; i < nops; ++i) A.ops[i] = ops[i];
     A.group_by_k = 1;
@@ -184,6 +187,8 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
     const size_t N = (size_t)1 << d->n, L = 2 * N, S = d->max_batch > 0 ? d->max_batch : 1;
     size_t b = 0;
     b += L * (sizeof(c128) + sizeof(int) + 2 * sizeof(double) + sizeof(int));
+    b += ztab_bytes(d->n, (int)L, PREC_F64);  // twiddle tables
+    if (d->precision != ADRT_PREC_F64) b += ztab_bytes(d->n, (int)L, PREC_F32);
     if (d->deg) {
         b += cheb_table_bytes((int)L, d->deg);  // Chebyshev weight table
     } else {
@@ -258,6 +263,11 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     if ((st = alloc((void**)&p->c, sizeof(double) * L)) || (st = up(p->c, cs, sizeof(double) * L))) return fail(st);
     if ((st = alloc(&p->wtab, cheb_table_bytes(L, degs))) ||
         (st = cheb_table_fill(L, degs, p->deg, p->a, p->c, p->wtab, nullptr, &p->wcoef, &p->wcoff)))
+        return fail(st);
+    if ((st = alloc(&p->ztab64, ztab_bytes(n, L, PREC_F64))) || (st = ztab_fill(n, L, p->z, PREC_F64, p->ztab64, nullptr)))
+        return fail(st);
+    if (p->d.precision == PREC_F32 &&
+        ((st = alloc(&p->ztab32, ztab_bytes(n, L, PREC_F32))) || (st = ztab_fill(n, L, p->z, PREC_F32, p->ztab32, nullptr))))
         return fail(st);
     if (hipStreamSynchronize(nullptr) != hipSuccess) {
         set_error("plan_create: Chebyshev weight table failed");

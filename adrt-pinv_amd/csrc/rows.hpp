@@ -42,6 +42,9 @@ struct RowsArgs {
     // (cheb_table_fill); the register kernel takes op lists with P ops only when they are given
     const c128* wcoef = nullptr;
     const int* wcoff = nullptr;
+    // z^t of parameter row k for t = 0 .. N/2 at ztab + k (N/2 + 1), in the compute type (ztab_fill); null: the
+    // register kernel builds the powers itself
+    const void* ztab = nullptr;
     // dense rows of the pass for the register kernel to compute as MFMA tiles beside the rows (n_dense = 0: none);
     // requires skip_dense and Z distinct from Y
     DenseArgs dn{};
@@ -63,6 +66,25 @@ int default_loge(int n, int prec);
 // (w_1 = 1 / (1 - c^2 / 2), w_j = 1 / (1 - c^2 w_(j-1) / 4), the reference's cpp/oblique.cpp:86-98), max(deg[k], 1)
 // entries per row.  deg_host is the host copy of d_deg; the weights are computed on the stream, the offsets are
 // copied before return.
+// z^(2^b), b < logn, by repeated squaring, and z^t as the product of those of t's set bits (in double): the register
+// kernel's twiddles, computed the same way by the kernel itself and by ztab_fill.
+__device__ __forceinline__ void ztab_powers(c128 z, int logn, c128* p2) {
+    c128 p = z;
+    for (int j = 0; j < logn; ++j) {
+        p2[j] = p;
+        p = mul(p, p);
+    }
+}
+__device__ __forceinline__ c128 ztab_entry(const c128* p2, int logn, int t) {
+    c128 w = mk(1.0, 0.0);
+    for (int b = 0; b < logn; ++b)
+        if ((t >> b) & 1) w = mul(w, p2[b]);
+    return w;
+}
+// Fills the (K, N/2 + 1) table of z_k^t in the precision's type (complex128 for PREC_F64, complex64 otherwise).
+size_t ztab_bytes(int n, int K, int prec);
+int ztab_fill(int n, int K, const c128* d_z, int prec, void* d_tab, hipStream_t st);
+
 size_t cheb_table_bytes(int K, const int* deg_host);
 int cheb_table_fill(int K, const int* deg_host, const int* d_deg, const double* d_a, const double* d_c, void* d_table,
                     hipStream_t st, const c128** wcoef, const int** wcoff);

@@ -89,7 +89,34 @@ __global__ __launch_bounds__(256) void cheb_coef_kernel(int L, const int* deg, c
 
 namespace {
 size_t cheb_table_head(int K) { return ((sizeof(int) * (size_t)K + 255) / 256) * 256; }
+
+// One thread per (row k, power t): the same products the register kernel would form for its own table.
+template <typename T>
+__global__ __launch_bounds__(256) void ztab_kernel(int n, int K, const c128* z, cplx<T>* out) {
+    const long long H = (1LL << n) / 2 + 1;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)K * H) return;
+    const int k = (int)(i / H), t = (int)(i % H);
+    c128 p2[11];
+    ztab_powers(z[k], n, p2);
+    out[i] = cast<T>(ztab_entry(p2, n, t));
+}
 }  // namespace
+
+size_t ztab_bytes(int n, int K, int prec) {
+    return (size_t)K * (((size_t)1 << n) / 2 + 1) * (prec == PREC_F64 ? sizeof(c128) : sizeof(c64));
+}
+
+int ztab_fill(int n, int K, const c128* d_z, int prec, void* d_tab, hipStream_t st) {
+    ADRT_REQUIRE(n >= 1 && n <= 11 && K >= 1 && d_z && d_tab, "ztab_fill: bad argument");
+    const long long cnt = (long long)K * ((1LL << n) / 2 + 1);
+    const unsigned grid = (unsigned)((cnt + 255) / 256);
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(ztab_kernel<double>, dim3(grid), dim3(256), 0, st, n, K, d_z, (c128*)d_tab);
+    else
+        hipLaunchKernelGGL(ztab_kernel<float>, dim3(grid), dim3(256), 0, st, n, K, d_z, (c64*)d_tab);
+    return launch_check("ztab_kernel");
+}
 
 size_t cheb_table_bytes(int K, const int* deg_host) {
     size_t e = 0;

@@ -283,22 +283,15 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     const int k = (int)(row % A.period);
     const c128 zk = A.z[k];
 
-    // ---- table of z^t, t = 0 .. N/2, in the row's LDS buffer: products of z^(2^b) in double, rounded to T
-    {
+    // ---- table of z^t, t = 0 .. N/2, in the row's LDS buffer: copied from the plan's per-frequency table when there
+    // is one, else built here (products of z^(2^b) in double, rounded to T, ztab_entry)
+    if (A.ztab) {
+        const cplx<T>* zt = reinterpret_cast<const cplx<T>*>(A.ztab) + (size_t)k * (C::N / 2 + 1);
+        for (int t = lane; t <= C::N / 2; t += C::G) X[t] = zt[t];
+    } else {
         c128 p2[LOGN];
-        c128 p = zk;
-#pragma unroll
-        for (int j = 0; j < LOGN; ++j) {
-            p2[j] = p;
-            p = mul(p, p);
-        }
-        for (int t = lane; t <= C::N / 2; t += C::G) {
-            c128 w = mk(1.0, 0.0);
-#pragma unroll
-            for (int b = 0; b < LOGN; ++b)
-                if ((t >> b) & 1) w = mul(w, p2[b]);
-            X[t] = cast<T>(w);
-        }
+        ztab_powers(zk, LOGN, p2);
+        for (int t = lane; t <= C::N / 2; t += C::G) X[t] = cast<T>(ztab_entry(p2, LOGN, t));
     }
     __syncthreads();
     // ---- the lane's twiddle pairs (z^t, z^(t+1)) of every pass
