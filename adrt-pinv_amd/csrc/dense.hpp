@@ -1,8 +1,9 @@
 // One 16 x 16 output tile of the dense low-frequency GEMM on the FP64 matrix cores, shared by the stand-alone dense
 // kernel (dense.hip) and the dense workers of the register row kernel (rows_reg.hpp).
 //
-// C[r][i] = sum_j Y[r][j] WT[j][i] for 16 rows r of one dense frequency and 16 outputs i, complex, as four real
-// v_mfma_f64_16x16x4 products per k-step (C_re += Yr Wr - Yi Wi, C_im += Yr Wi + Yi Wr).  Operand layouts of the
+// C[r][i] = sum_j Y[r][j] WT[j][i] for 16 rows r of one dense frequency and 16 outputs i, complex, as three real
+// v_mfma_f64_16x16x4 products per k-step (Gauss: P1 += Yr Wr, P2 += Yi Wi, P3 += (Yr + Yi)(Wr + Wi), then
+// C_re = P1 - P2, C_im = P3 - P1 - P2; the sums Yr + Yi of float rows are exact in double).  Operand layouts of the
 // 16x16x4 f64 instruction (checked on gfx950): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
 // D[4 r + lane / 16][lane % 16] in accumulator element r.  The k-steps run in ascending order whatever U is, so
 // every caller gets the same bits for a row.
@@ -27,7 +28,7 @@ namespace dense {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // The tile of rows r0..r0+15 and outputs c0..c0+15 of dense slot `slot`, computed by one wave (lane 0..63).  U
-// k-steps (of 4) per load group, the next group's loads in flight while the current group's 4 U MFMAs run.  Zout
+// k-steps (of 4) per load group, the next group's loads in flight while the current group's 3 U MFMAs run.  Zout
 // must not alias Yin (other waves still read the frequency's rows).
 template <typename Tg, int U>
 __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0, int c0, int lane, const void* Yin,
@@ -40,7 +41,7 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
     const bool rv = row < D.SP;
     const cplx<Tg>* Arow = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv ? row : 0) * D.L + k) * N + kq;
     const cplx<Tg> zero = mk((Tg)0, (Tg)0);
-    v4d cre = v4d{0.0, 0.0, 0.0, 0.0}, cim = v4d{0.0, 0.0, 0.0, 0.0};
+    v4d p1 = v4d{0.0, 0.0, 0.0, 0.0}, p2 = v4d{0.0, 0.0, 0.0, 0.0}, p3 = v4d{0.0, 0.0, 0.0, 0.0};
     cplx<Tg> a[U], an[U];
     c128 b[U], bn[U];
 #pragma unroll
@@ -60,10 +61,9 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const double ar = (double)a[u].x, ai = (double)a[u].y;
-            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].x, cre, 0, 0, 0);
-            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, b[u].y, cre, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].y, cim, 0, 0, 0);
-            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[u].x, cim, 0, 0, 0);
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[u].y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, b[u].x + b[u].y, p3, 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -75,7 +75,7 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int sp = r0 + 4 * e + kq;
-        if (sp < D.SP) Z[((size_t)sp * D.L + k) * N + c0 + li] = cast<Tg>(mk(cre[e], cim[e]));
+        if (sp < D.SP) Z[((size_t)sp * D.L + k) * N + c0 + li] = cast<Tg>(mk(p1[e] - p2[e], (p3[e] - p1[e]) - p2[e]));
     }
 }
 

@@ -249,9 +249,10 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             for (int c = 0; c < 4; ++c) {
                 const bool in = (inm[e] >> c) & 1;
                 if constexpr (INCR) {
-                    const T d = om * fma(-a, g[c], yv[e][c]) + om1 * xq[e][c];
-                    xq[e][c] = in ? d : (T)0;
-                    xc[e][c] = in ? xc[e][c] + d : (T)0;
+                    // a pixel outside the image has y = 0 and x = 0 from the start: masking its increment keeps it 0
+                    const T d = in ? om * fma(-a, g[c], yv[e][c]) + om1 * xq[e][c] : (T)0;
+                    xq[e][c] = d;
+                    xc[e][c] = xc[e][c] + d;
                 } else {
                     const T xo = xc[e][c];
                     const T xn = in ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : (T)0;
