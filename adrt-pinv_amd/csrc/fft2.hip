@@ -36,6 +36,9 @@ __global__ __launch_bounds__(L1* C) void fft2_kernel(int n, int nsino, const voi
     const int c = (lb % gx) * C + cl;
     const int pair = (lb / gx) % 2, s = lb / (2 * gx);
     for (int i = t; i < L / 2; i += L1 * C) tw[i] = twg[i];
+    const double* x = MODE == 0 ? reinterpret_cast<const double*>(src) + (size_t)s * N * N : nullptr;
+    T* stg = reinterpret_cast<T*>(S);  // mode 0: the transposed strip's rows, before the exchange uses S
+    if (MODE == 0) stage_rows<T, 16 * L1, C, L1 * C>(x, pair, c - cl, stg, t);  // N = 16 L1
     __syncthreads();
     cplx<T> w32[16];
 #pragma unroll
@@ -44,17 +47,13 @@ __global__ __launch_bounds__(L1* C) void fft2_kernel(int n, int nsino, const voi
     // ---- step 1: load x[h1 + L1 h2], 32-point DFT over h2, twiddle W_L^{h1 k2}
     cplx<T> v[32];
     if (MODE == 0) {
-        const double* x = reinterpret_cast<const double*>(src) + (size_t)s * N * N;
 #pragma unroll
         for (int h2 = 0; h2 < 32; ++h2) {
             const int h = h1 + L1 * h2;
-            if (h2 < 16) {  // h < N (N = 16 L1); the zero padding rows h >= N contribute nothing
-                v[h2].x = (T)image_strip2(x, N, 2 * pair, h, c);
-                v[h2].y = (T)image_strip2(x, N, 2 * pair + 1, h, c);
-            } else {
-                v[h2] = mk((T)0, (T)0);
-            }
+            // h < N (N = 16 L1); the zero padding rows h >= N contribute nothing
+            v[h2] = h2 < 16 ? strip_pair<T>(x, N, pair, h, c, cl, C, stg) : mk((T)0, (T)0);
         }
+        __syncthreads();  // every thread has its column before step 1 overwrites the staged rows
     } else if (MODE == 1) {
         const size_t qsz = (size_t)(2 * N - 1) * N;
         const double* ba = reinterpret_cast<const double*>(src) + ((size_t)s * 4 + 2 * pair) * qsz;

@@ -58,5 +58,32 @@ __device__ __forceinline__ double image_strip2(const double* x, int N, int q, in
     }
 }
 
+// Mode-0 packing of one pair's two strips for a block's C columns c0 .. c0 + C - 1.  The pair's transposed strip
+// (q = 0 for pair 0, q = 3 for pair 1) is C whole image rows: stage_rows copies them into LDS with coalesced loads
+// (C rows of N + 1 values, rounded to T as the strip read would), and strip_pair takes that strip's value from there
+// and the other strip's x[.][c] from memory, which is coalesced across the block's columns.
+template <typename T, int N, int C, int NT>
+__device__ __forceinline__ void stage_rows(const double* x, int pair, int c0, T* stg, int t) {
+    static_assert((C * N) % NT == 0, "stage_rows: the rows must split evenly over the threads");
+    constexpr int PER = C * N / NT;
+    const int row0 = pair == 0 ? c0 : N - c0 - C;
+    const double* src = x + (size_t)row0 * N;  // C consecutive rows: one contiguous span
+    double val[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) val[e] = src[t + NT * e];  // every load in flight before the first store
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int i = t + NT * e, r = i / N, j = i % N;
+        stg[r * (N + 1) + j] = (T)val[e];
+    }
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> strip_pair(const double* x, int N, int pair, int h, int c, int cl, int C,
+                                              const T* stg) {
+    const T tv = stg[(pair == 0 ? cl : C - 1 - cl) * (N + 1) + (N - 1 - h)];
+    const T nv = (T)image_strip2(x, N, pair == 0 ? 1 : 2, h, c);
+    return pair == 0 ? mk(tv, nv) : mk(nv, tv);
+}
+
 }  // namespace fftreg
 }  // namespace adrt
