@@ -79,9 +79,11 @@ __global__ void lorr_kernel(int n, const double* y, double* out, long long count
 // a cell (i, j), G_1 x = 8 x + [1 2 1] down the partner column j^1 (rows i-1..i+1) + [1 2 1] along the partner row i^1
 // (columns j-1..j+1) (fasthybrid.py:81-98); inside a 2 x 2 cell both partners are in the cell, so a step needs only
 // the four neighbouring cells: the bottom row of the cell above, the top row of the cell below, the right column of
-// the cell to the left and the left column of the cell to the right.  LDS holds the iterate cell-major, one 4-vector
-// per cell (x00, x01, x10, x11), so each neighbour is one aligned 16 / 32-byte read, double-buffered (step k reads
-// buffer (k-1)&1 and writes k&1, one barrier per step).  Cells are numbered ring by ring: the tile's cells first,
+// the cell to the left and the left column of the cell to the right.  LDS holds the iterate as four planes, one per
+// cell position (x00, x01, x10, x11), each indexed by cell: the lanes of a wave take consecutive cells, so every
+// neighbour component they read is a run of consecutive words, free of bank conflicts (a cell-major 4-vector layout
+// made each such read 4- to 8-way conflicted).  Double-buffered: step k reads buffer (k-1)&1 and writes k&1, one
+// barrier per step.  Cells are numbered ring by ring: the tile's cells first,
 // then the rings of halo cells at Chebyshev distance 1, 2, ... cells, and cell b belongs to thread b % THREADS.  The
 // cells a step must compute (rings up to (deg - k) / 2) are then a prefix of that numbering, so whole waves drop out
 // as the square shrinks.  Cells outside the image and a one-cell border around the region hold zero, the stencil's
@@ -102,10 +104,6 @@ struct G1Args {
     double om[32];     // inner omega_k, k = 1..deg-1
 };
 
-template <typename T>
-struct alignas(4 * sizeof(T)) Cell4 {
-    T v[4];  // x00, x01, x10, x11
-};
 
 template <typename T, int TILE, int HALO>
 struct G1Cfg {
@@ -117,7 +115,7 @@ struct G1Cfg {
     static constexpr int BUF = CP * CP;
     static constexpr int NB = RB * RB;
     static constexpr int BPT = (NB + G1_THREADS - 1) / G1_THREADS;
-    static constexpr size_t LDS = 2 * (size_t)BUF * sizeof(Cell4<T>);
+    static constexpr size_t LDS = 2 * 4 * (size_t)BUF * sizeof(T);  // two buffers of four planes
     // number of cells in the tile and its first j rings
     __host__ __device__ static constexpr int upto(int j) { return (TB + 2 * j) * (TB + 2 * j); }
 };
@@ -158,7 +156,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     using C = G1Cfg<T, TILE, HALO>;
     constexpr int CP = C::CP, BUF = C::BUF, NB = C::NB, BPT = C::BPT, TB = C::TB, HB = C::HB;
     extern __shared__ __align__(16) unsigned char g1_smem[];
-    Cell4<T>* X = reinterpret_cast<Cell4<T>*>(g1_smem);  // two buffers of (CP, CP) cells
+    T* X = reinterpret_cast<T*>(g1_smem);  // two buffers of four (CP, CP) planes: plane q of buffer u at X + (4u + q) BUF
     const int N = 1 << A.n;
     const int H = A.deg - 1;  // <= HALO
     // the tiles of one image on one XCD: neighbouring tiles' halos overlap, and are then read from that L2
@@ -169,7 +167,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     const int gi0 = ((lb / nt) % nt) * TILE - HALO, gj0 = (lb % nt) * TILE - HALO;
     const size_t img = (size_t)s * N * N;
     const T a = (T)(2.0 / 20.0);
-    for (int i = threadIdx.x; i < 2 * BUF; i += G1_THREADS) X[i] = Cell4<T>{{(T)0, (T)0, (T)0, (T)0}};
+    for (int i = threadIdx.x; i < 8 * BUF; i += G1_THREADS) X[i] = (T)0;
     __syncthreads();
     // float carries the increment d_k = x_k - x_(k-1) in xq and a y in yv: x_(k+1) = x_k + d_(k+1) with
     // d_(k+1) = w a (y - G_1 x_k) + (w - 1) d_k (the same recurrence, fewer instructions); double keeps the reference's
@@ -199,7 +197,10 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
                 xq[e][c] = xc[e][c];  // the first increment x_0 - 0
             }
         }
-        if (b < NB) X[lds[e]] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
+        if (b < NB) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) X[c * BUF + lds[e]] = xc[e][c];
+        }
     }
     __syncthreads();
     // the outer update's x and x_prev of the tile's cells (the first TB^2 cells, slots e < EPI), loaded now so that
@@ -223,8 +224,8 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     }
     for (int k = 1; k <= H; ++k) {
         const T om = (T)A.om[k - 1], om1 = (T)(A.om[k - 1] - 1.0);
-        const Cell4<T>* Xr = X + ((k - 1) & 1) * BUF;
-        Cell4<T>* Xw = X + (k & 1) * BUF;
+        const T* Xr = X + ((k - 1) & 1) * 4 * BUF;
+        T* Xw = X + (k & 1) * 4 * BUF;
         // the cells that must be exact after step k: rings j with 2 j - 1 <= H - k, a prefix of the numbering
         const int jmax = min((H - k + 1) / 2, HB);
         const int active = C::upto(jmax);
@@ -233,11 +234,10 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             const int b = threadIdx.x + G1_THREADS * e;
             if (b >= active) continue;
             const int o = lds[e];
-            const Cell4<T> up = Xr[o - CP], dn = Xr[o + CP], lf = Xr[o - 1], rt = Xr[o + 1];
-            const T up0 = up.v[2], up1 = up.v[3];  // x[r0 - 1][c0], x[r0 - 1][c0 + 1]
-            const T dn0 = dn.v[0], dn1 = dn.v[1];  // x[r0 + 2][c0], x[r0 + 2][c0 + 1]
-            const T lf0 = lf.v[1], lf1 = lf.v[3];  // x[r0][c0 - 1], x[r0 + 1][c0 - 1]
-            const T rt0 = rt.v[0], rt1 = rt.v[2];  // x[r0][c0 + 2], x[r0 + 1][c0 + 2]
+            const T up0 = Xr[2 * BUF + o - CP], up1 = Xr[3 * BUF + o - CP];  // x[r0 - 1][c0], x[r0 - 1][c0 + 1]
+            const T dn0 = Xr[o + CP], dn1 = Xr[BUF + o + CP];                // x[r0 + 2][c0], x[r0 + 2][c0 + 1]
+            const T lf0 = Xr[BUF + o - 1], lf1 = Xr[3 * BUF + o - 1];       // x[r0][c0 - 1], x[r0 + 1][c0 - 1]
+            const T rt0 = Xr[o + 1], rt1 = Xr[2 * BUF + o + 1];              // x[r0][c0 + 2], x[r0 + 1][c0 + 2]
             const T x00 = xc[e][0], x01 = xc[e][1], x10 = xc[e][2], x11 = xc[e][3];
             const T two = (T)2, eight = (T)8;
             T g[4];
@@ -260,7 +260,8 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
                     xc[e][c] = xn;
                 }
             }
-            Xw[o] = Cell4<T>{{xc[e][0], xc[e][1], xc[e][2], xc[e][3]}};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Xw[c * BUF + o] = xc[e][c];
         }
         __syncthreads();
     }
