@@ -11,6 +11,11 @@ With the mapping fixed it then places the dense low-frequency GEMM of the pass (
 inside the row kernel's launch, at most 1, 2 or 4 per SIMD, or as a launch of its own after the rows
 (adrt_dense_config_set).  Few dense waves leave the vector ALUs to the rows; more hide the GEMM's load latency better
 when the dense work is large (N >= 1024, where a third of the frequencies can be dense).
+
+An FP32 plan with FP64 refinement passes (refine >= 1) runs those passes at the FP32 plan's batch depth, where the
+best FP64 mapping can differ from a plain FP64 plan's: the tuner times them too, on the rows an apply's last (FP64)
+pass leaves, and installs the choice as class 2 of adrt_rows_config_set (measured at N=256 with 64 sinograms: 16
+values per lane, 3% off the whole apply against the default 4).
 """
 
 import json
@@ -24,12 +29,15 @@ DENSE_CANDIDATES = (-1, 1, 2, 4)
 
 
 class TuneResult(tuple):
-    """(loge, {loge: ms}); the dense placement chosen after it is in .dense, its timings in .dense_times."""
+    """(loge, {loge: ms}); the dense placement chosen after it is in .dense, its timings in .dense_times; for an FP32
+    plan with FP64 refinement passes, their row mapping in .f64 (None otherwise), its timings in .f64_times."""
 
-    def __new__(cls, loge, times, dense=0, dense_times=None):
+    def __new__(cls, loge, times, dense=0, dense_times=None, f64=None, f64_times=None):
         r = super().__new__(cls, (loge, times))
         r.dense = dense
         r.dense_times = dense_times or {}
+        r.f64 = f64
+        r.f64_times = f64_times or {}
         return r
 
 
@@ -68,14 +76,15 @@ def apply_table(table=None):
     lib = L.load()
     for (n, prec), (loge, dense) in (table if table is not None else load_table()).items():
         lib.adrt_rows_config_set(n, prec, loge)
-        lib.adrt_dense_config_set(n, prec, dense)
+        if prec < 2:  # class 2 (FP64 refinement passes of FP32 plans) shares the FP64 dense placement
+            lib.adrt_dense_config_set(n, prec, dense)
 
 
-def _time_pass(lib, plan, S, st, e0, e1, reps):
-    L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")  # warm-up
+def _time_pass(lib, plan, S, st, e0, e1, reps, which=1):
+    L.check(lib.adrt_plan_rows_pass(plan._plan, which, S, st.handle), "rows_pass")  # warm-up
     e0.record(st)
     for _ in range(reps):
-        L.check(lib.adrt_plan_rows_pass(plan._plan, 1, S, st.handle), "rows_pass")
+        L.check(lib.adrt_plan_rows_pass(plan._plan, which, S, st.handle), "rows_pass")
     e1.record(st)
     return e0.elapsed_ms(e1) / reps
 
@@ -111,8 +120,23 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=D
             dense_times[m] = _time_pass(lib, plan, S, st, e0, e1, reps)
         dense = min(dense_times, key=dense_times.get)
     lib.adrt_dense_config_set(n, prec, dense)
+    # the FP64 refinement passes of an FP32 plan (class 2): an apply leaves the rows of its last, FP64 pass
+    f64, f64_times = None, {}
+    if plan.precision != "fp64" and getattr(plan, "refine", 0) > 0:
+        db = L.DeviceArray((S, 4, 2 * plan.N - 1, plan.N))
+        db.zero(st)
+        plan.apply_device(db, dy, S, st, count=False)
+        for c in candidates:
+            if c > 0 and (1 << c) > plan.N:
+                continue
+            L.check(lib.adrt_rows_config_set(n, 2, c), "rows_config_set")
+            f64_times[c] = _time_pass(lib, plan, S, st, e0, e1, reps, which=3)
+        f64 = min(f64_times, key=f64_times.get)
+        lib.adrt_rows_config_set(n, 2, f64)
     if persist:
         table = load_table()
         table[(n, prec)] = (best, dense)
+        if f64 is not None:
+            table[(n, 2)] = (f64, 0)
         save_table(table)
-    return TuneResult(best, times, dense, dense_times)
+    return TuneResult(best, times, dense, dense_times, f64, f64_times)

@@ -91,6 +91,7 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
     for (int i = 0; i < nops; ++i) A.ops[i] = ops[i];
     A.group_by_k = 1;
     A.SP = 2 * S;
+    A.cfg_class = (prec == PREC_F64 && p->d.precision == PREC_F32) ? 2 : prec;
     A.korder = p->korder;
     A.comp = prec == PREC_F32 ? p->comp : 0;
     if (WT && p->d.n_dense > 0) {
@@ -458,10 +459,12 @@ int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, vo
 int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream) {
     auto* p = (Plan*)plan;
     ADRT_TRY(check_batch(p, S));
-    ADRT_REQUIRE(which == 0 || which == 1, "plan_rows_pass: which must be 0 (backproject ops) or 1 (operator ops)");
+    ADRT_REQUIRE(which >= 0 && which <= 3, "plan_rows_pass: which must be 0 / 1 (backproject / operator ops) or 2 / 3 "
+                                           "(the same in FP64)");
     if (S == 0) return OK;
-    return which == 0 ? rows_pass(p, p->d.precision, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2, p->WbackT)
-                      : rows_pass(p, p->d.precision, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2, p->WopT);
+    const int prec = which >= 2 ? PREC_F64 : p->d.precision;
+    return (which % 2) == 0 ? rows_pass(p, prec, p->ops_back, p->nback, S, (hipStream_t)stream, p->Y2, p->WbackT)
+                            : rows_pass(p, prec, p->ops_op, p->nop, S, (hipStream_t)stream, p->Y2, p->WopT);
 }
 
 int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end) {

@@ -275,10 +275,11 @@ int dispatch(const RowsArgs& A, hipStream_t s) {
 
 }  // namespace
 
-// Per-(n, precision) choice of the register kernel's elements per lane: 0 = built-in default, -1 = generic kernel,
-// 2..4 = log2 elements per lane.  Set by the autotuner (adrt_rows_config_set) or the environment variable
-// ADRT_ROWS_LOGE (one value for every size, read once).
-static int g_rows_cfg[2][12];
+// Per-(n, class) choice of the register kernel's elements per lane: 0 = built-in default, -1 = generic kernel,
+// 2..4 = log2 elements per lane.  Classes: 0 FP64 passes, 1 FP32 passes, 2 the FP64 refinement passes of FP32 plans
+// (their batch depth differs from a plain FP64 plan's; unset, they follow class 0).  Set by the autotuner
+// (adrt_rows_config_set) or the environment variable ADRT_ROWS_LOGE (one value for every size, read once).
+static int g_rows_cfg[3][12];
 static int g_rows_env = -100;
 
 int rows_config_get(int n, int prec) {
@@ -286,13 +287,14 @@ int rows_config_get(int n, int prec) {
         const char* e = getenv("ADRT_ROWS_LOGE");
         g_rows_env = e ? atoi(e) : 0;
     }
-    if (n < 0 || n > 11 || prec < 0 || prec > 1) return 0;
+    if (n < 0 || n > 11 || prec < 0 || prec > 2) return 0;
     if (g_rows_cfg[prec][n] != 0) return g_rows_cfg[prec][n];
+    if (prec == 2) return rows_config_get(n, 0);
     return g_rows_env;
 }
 
 int rows_config_set(int n, int prec, int loge) {
-    ADRT_REQUIRE(n >= 0 && n <= 11 && (prec == 0 || prec == 1), "rows_config_set: bad n or precision");
+    ADRT_REQUIRE(n >= 0 && n <= 11 && prec >= 0 && prec <= 2, "rows_config_set: bad n or precision class");
     ADRT_REQUIRE(loge >= -1 && loge <= 4 && loge != 1, "rows_config_set: loge must be -1, 0, 2, 3 or 4");
     g_rows_cfg[prec][n] = loge;
     return OK;
@@ -329,8 +331,9 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
     ADRT_REQUIRE(A.period >= 1, "rows: period must be positive");
     ADRT_REQUIRE(A.dn.n_dense == 0 || (A.skip_dense && A.Z != A.Y && dense_done),
                  "rows: dense tiles need skip_dense, distinct input and output rows and a completion flag");
-    if (A.loge >= 0 && rows_config_get(A.n, prec) >= 0) {
-        const int cfg = rows_config_get(A.n, prec);
+    const int cls = A.cfg_class >= 0 ? A.cfg_class : prec;
+    if (A.loge >= 0 && rows_config_get(A.n, cls) >= 0) {
+        const int cfg = rows_config_get(A.n, cls);
         const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));
         if (loge > 0) {
             const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, prec) >= 0;

@@ -303,7 +303,9 @@ def main():
     tuned = pkg("autotune").tune(H32, B, persist=False)
     loge, tune_ms = tuned
     log(f"autotuned row mapping: loge={loge} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tune_ms.items())}); "
-        f"dense placement {tuned.dense} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.dense_times.items())})")
+        f"dense placement {tuned.dense} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.dense_times.items())}); "
+        f"FP64 refinement passes loge={tuned.f64} "
+        f"({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.f64_times.items())})")
 
     # inputs: ellipse phantoms (seeds 1000+i) through the GPU forward ADRT, plus 1% noise (seeds 2000+i)
     Bn = make_sinos(n, B, 1000, 2000)
@@ -416,7 +418,8 @@ def main():
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
                    "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
                    "sum_deg_over_N": float(H64.deg.sum() / N), "n_dense": H64.eng.n_dense,
-                   "row_mapping_loge": int(loge), "dense_placement": int(tuned.dense)},
+                   "row_mapping_loge": int(loge), "dense_placement": int(tuned.dense),
+                   "fp64_refine_row_mapping_loge": tuned.f64},
         "plain_fp64_inverses_per_s": plain_ips,
         "speedup_vs_plain_fp64": ips / plain_ips,
         "relerr_fp32_vs_fp64_max": float(rel.max()),

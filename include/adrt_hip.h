@@ -88,8 +88,9 @@ int adrt_rows_plan_destroy(adrt_rows_plan* plan);
 int adrt_rows_run(adrt_rows_plan* plan, const void* dY, void* dZ, long long rows, const int* ops, int nops,
                   void* stream);
 
-/* Row-kernel variant per (n, precision): 0 = built-in default, -1 = generic LDS kernel, 2..4 = register kernel with
- * 2^loge values per lane.  Chosen by the autotuner; the environment variable ADRT_ROWS_LOGE sets every size. */
+/* Row-kernel variant per (n, class): 0 = built-in default, -1 = generic LDS kernel, 2..4 = register kernel with
+ * 2^loge values per lane.  Class 0: FP64 passes, 1: FP32 passes, 2: the FP64 refinement passes of FP32 plans (unset:
+ * as class 0).  Chosen by the autotuner; the environment variable ADRT_ROWS_LOGE sets every size. */
 int adrt_rows_config_set(int n, int precision, int loge);
 int adrt_rows_config_get(int n, int precision);
 
@@ -156,8 +157,9 @@ int adrt_plan_backproject(adrt_plan* plan, const double* d_b, double* d_y, int S
 int adrt_plan_operator(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
 /* y = P x: g1_inv / lorr_inv (hybrid) or the identity (fasthybrid.py:101-130) */
 int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, void* stream);
-/* the row kernel alone on the rows of the plan's last packed pass (which: 0 backproject ops, 1 operator ops), written
- * to a second buffer: the dominant kernel of an apply, for timing */
+/* the row kernel alone on the rows of the plan's last packed pass (which: 0 backproject ops, 1 operator ops, in the
+ * plan's precision; 2 / 3 the same in FP64, on rows an FP64 pass left, e.g. after an apply of an FP32 plan with
+ * refine >= 1), written to a second buffer: the dominant kernel of an apply, for timing and tuning */
 int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream);
 /* every later launch of the plan's row kernel is bracketed by records of the two events (NULL, NULL: stop), so the
  * kernel alone can be timed inside a pass; not for a plan whose work is being captured into a graph */

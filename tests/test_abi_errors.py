@@ -89,3 +89,19 @@ def test_dense_config_validates(L):
             assert lib.adrt_dense_config_get(8, 1) == m
     finally:
         lib.adrt_dense_config_set(8, 1, old)
+
+
+def test_rows_config_classes(L):
+    """Class 2 (the FP64 refinement passes of FP32 plans) follows class 0 until set; classes above 2 are errors."""
+    lib = L.load()
+    assert lib.adrt_rows_config_set(8, 3, 2) == -1 and "class" in err(L)
+    old0, old2 = lib.adrt_rows_config_get(8, 0), lib.adrt_rows_config_get(8, 2)
+    try:
+        assert lib.adrt_rows_config_set(8, 2, 0) == 0
+        assert lib.adrt_rows_config_set(8, 0, 3) == 0
+        assert lib.adrt_rows_config_get(8, 2) == 3
+        assert lib.adrt_rows_config_set(8, 2, 4) == 0
+        assert lib.adrt_rows_config_get(8, 2) == 4 and lib.adrt_rows_config_get(8, 0) == 3
+    finally:
+        lib.adrt_rows_config_set(8, 0, old0)
+        lib.adrt_rows_config_set(8, 2, old2 if old2 != old0 else 0)
