@@ -13,16 +13,19 @@
 // Those are the only ranges the inverses use (fastops.py:281-285, 330-335), so the passes are straight-line code.
 // Each lane keeps the pair (z^t, z^(t+1)) of every twiddle slot it uses in registers: the forward pair is
 // (f0 + z^t f1, f0 + z^(t+1) f1), the adjoint (g0 + g1, conj(z^t) g0 + conj(z^(t+1)) g1), two FMAs per real component,
-// and no twiddle product is recomputed inside the Chebyshev loop.  The twiddles come from a table of z^t, t <= N/2,
-// that the row's lanes build in LDS at start (double products of z^(2^b), rounded to the compute type).
+// and no twiddle product is recomputed inside the Chebyshev loop.  The twiddles come from a table of z^t, t <= N/2, in
+// the row's LDS at start: copied from the plan's per-frequency table (ztab_fill), or built by the row's lanes
+// (double products of z^(2^b), rounded to the compute type) when the caller has none.  The innermost level n runs
+// forward and adjoint as one halved step on h = (z^t + z^(t+1)) / 2 (inner_half), and the Chebyshev step weights come
+// from the plan's table (cheb_table_fill).
 //
 // Rows of one block run the same macro-op program; P rows with different degrees iterate to the block's largest
 // degree with the update predicated off, so every barrier is block-uniform.  The plan maps blocks to rows of one
 // frequency (equal degree) in descending-degree order.
 //
 // A pass with dense rows (RowsArgs::dn) puts its dense GEMM tiles in the same launch: the leading blocks are dense
-// workers, one wave per SIMD at most, each taking a contiguous share of the 16 x 16 MFMA tiles (dense.hpp), while
-// the row blocks behind them fill the remaining wave slots.  The tiles' loads and matrix-core work then overlap the
+// workers, at most 1, 2 or 4 waves per SIMD (dense_config_get, chosen by the autotuner), each taking a contiguous
+// share of the 16 x 16 MFMA tiles (dense.hpp), while the row blocks behind them fill the remaining wave slots.  The tiles' loads and matrix-core work then overlap the
 // rows' vector work instead of running as a launch of their own after it.
 #pragma once
 #include "rows.hpp"
