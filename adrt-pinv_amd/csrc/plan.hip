@@ -2,6 +2,8 @@
 // (fastops.py:175-338 on the device), hipGraph capture, and the forward / transpose ADRT.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/adrt_hip.h"
@@ -9,6 +11,32 @@
 #include "rows.hpp"
 
 namespace adrt {
+
+// A caller's stream and device-memory budget (adrt_ctx_*): plans and buffers charge their bytes to it.
+struct Context {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    unsigned long long budget = 0;  // 0: no limit
+    unsigned long long used = 0;
+    long long plans = 0;            // live plans created in the context
+    std::unordered_map<void*, unsigned long long> bufs;  // live buffers and their sizes
+
+    // reserves `bytes` against the budget, or fails with E_NOMEM and leaves the usage unchanged
+    int charge(unsigned long long bytes, const char* what) {
+        std::lock_guard<std::mutex> g(mu);
+        if (budget != 0 && (bytes > budget || used > budget - bytes)) {
+            set_error(std::string(what) + ": " + std::to_string(bytes) + " bytes exceed the context's budget (" +
+                      std::to_string(budget - used) + " of " + std::to_string(budget) + " left)");
+            return E_NOMEM;
+        }
+        used += bytes;
+        return OK;
+    }
+    void release(unsigned long long bytes) {
+        std::lock_guard<std::mutex> g(mu);
+        used -= bytes < used ? bytes : used;
+    }
+};
 
 struct Plan {
     adrt_plan_desc d{};
@@ -49,6 +77,8 @@ struct Plan {
     int comp = 0;  // compensated FP32 Chebyshev accumulation (desc precision ADRT_PREC_F32)
     hipEvent_t ev_rows0 = nullptr, ev_rows1 = nullptr;  // bracket every row-kernel launch when set (timing)
     bool borrowed = false;  // dFT / WopT / WbackT belong to the caller (desc dense_on_device == 2)
+    Context* ctx = nullptr;        // the context the plan's bytes are charged to (adrt_plan_create_in), or null
+    unsigned long long charged = 0;
 };
 
 namespace {
@@ -415,7 +445,107 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
 }
 
 int adrt_plan_destroy(adrt_plan* plan) {
-    free_plan((Plan*)plan);
+    auto* p = (Plan*)plan;
+    if (p && p->ctx) {
+        p->ctx->release(p->charged);
+        std::lock_guard<std::mutex> g(p->ctx->mu);
+        --p->ctx->plans;
+    }
+    free_plan(p);
+    return OK;
+}
+
+int adrt_ctx_create(adrt_ctx** out, unsigned long long budget_bytes) {
+    ADRT_REQUIRE(out, "ctx_create: null out pointer");
+    *out = nullptr;
+    auto* c = new Context();
+    c->budget = budget_bytes;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("ctx_create: stream creation failed");
+        return E_HIP;
+    }
+    *out = (adrt_ctx*)c;
+    return OK;
+}
+
+int adrt_ctx_destroy(adrt_ctx* ctx) {
+    auto* c = (Context*)ctx;
+    if (!c) return OK;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (c->plans != 0 || !c->bufs.empty()) {
+            set_error("ctx_destroy: " + std::to_string(c->plans) + " plans and " + std::to_string(c->bufs.size()) +
+                      " buffers of the context are still alive");
+            return E_STATE;
+        }
+    }
+    ADRT_HIP(hipStreamSynchronize(c->stream));
+    ADRT_HIP(hipStreamDestroy(c->stream));
+    delete c;
+    return OK;
+}
+
+void* adrt_ctx_stream(adrt_ctx* ctx) { return ctx ? (void*)((Context*)ctx)->stream : nullptr; }
+
+int adrt_ctx_usage(adrt_ctx* ctx, unsigned long long* used_bytes, unsigned long long* budget_bytes) {
+    auto* c = (Context*)ctx;
+    ADRT_REQUIRE(c, "ctx_usage: null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (used_bytes) *used_bytes = c->used;
+    if (budget_bytes) *budget_bytes = c->budget;
+    return OK;
+}
+
+int adrt_ctx_malloc(adrt_ctx* ctx, void** ptr, unsigned long long bytes) {
+    auto* c = (Context*)ctx;
+    ADRT_REQUIRE(c && ptr, "ctx_malloc: null context or pointer");
+    *ptr = nullptr;
+    ADRT_TRY(c->charge(bytes, "ctx_malloc"));
+    const int st = adrt_malloc(ptr, bytes);
+    if (st != OK) {
+        c->release(bytes);
+        return st;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    c->bufs[*ptr] = bytes;
+    return OK;
+}
+
+int adrt_ctx_free(adrt_ctx* ctx, void* ptr) {
+    auto* c = (Context*)ctx;
+    ADRT_REQUIRE(c, "ctx_free: null context");
+    if (!ptr) return OK;
+    unsigned long long bytes = 0;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        auto it = c->bufs.find(ptr);
+        ADRT_REQUIRE(it != c->bufs.end(), "ctx_free: the pointer is not a buffer of this context");
+        bytes = it->second;
+        c->bufs.erase(it);
+    }
+    c->release(bytes);
+    return adrt_free(ptr);
+}
+
+int adrt_plan_create_in(adrt_ctx* ctx, adrt_plan** out, const adrt_plan_desc* desc) {
+    auto* c = (Context*)ctx;
+    ADRT_REQUIRE(c, "plan_create_in: null context");
+    ADRT_REQUIRE(out && desc, "plan_create_in: null pointer");
+    *out = nullptr;
+    const unsigned long long bytes = adrt_plan_bytes(desc);
+    if (bytes == 0) return adrt_plan_create(out, desc);  // a size out of range: plan_create names the error
+    ADRT_TRY(c->charge(bytes, "plan_create_in"));
+    const int st = adrt_plan_create(out, desc);
+    if (st != OK) {
+        c->release(bytes);
+        return st;
+    }
+    auto* p = (Plan*)*out;
+    p->ctx = c;
+    p->charged = bytes;
+    std::lock_guard<std::mutex> g(c->mu);
+    ++c->plans;
     return OK;
 }
 

@@ -13,6 +13,7 @@ eigensolve) touches the host.  Extra keywords:
   max_batch  sinograms per device call (``apply_batch``)
   outer      (lo, hi, D) to skip the eigensolve (e.g. to reuse the reference's setup scalars)
   refine     with precision 'fp32': the backprojection and the last ``refine`` outer steps run in FP64
+  ctx        a lib.Context whose device-memory budget the plan is charged to
 """
 
 import ctypes
@@ -26,6 +27,9 @@ from . import spectral
 
 
 class _PlanBase:
+    """Shared part of the GPU plans.  ctx: an optional lib.Context; the plan's device bytes (adrt_plan_bytes) and its
+    staging arrays are then charged to that context's budget, and creation fails with HipError (-3) beyond it."""
+
     KIND = None
 
     def _setup_common(self, n, tau_fac, eps_in, threads):
@@ -77,12 +81,16 @@ class _PlanBase:
         d.dense_on_device = 2
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
-        L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
+        ctx = getattr(self, "ctx", None)
+        if ctx is None:
+            L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
+        else:
+            L.check(L.load().adrt_plan_create_in(ctx.handle, ctypes.byref(h), ctypes.byref(d)), "plan_create_in")
         self._plan = h.value
         N = self.N
-        self._dimg = L.DeviceArray((self.max_batch, N, N), np.float64)
-        self._dimg2 = L.DeviceArray((self.max_batch, N, N), np.float64)
-        self._dsino = L.DeviceArray((self.max_batch, 4, 2 * N - 1, N), np.float64)
+        self._dimg = L.DeviceArray((self.max_batch, N, N), np.float64, ctx=ctx)
+        self._dimg2 = L.DeviceArray((self.max_batch, N, N), np.float64, ctx=ctx)
+        self._dsino = L.DeviceArray((self.max_batch, 4, 2 * N - 1, N), np.float64, ctx=ctx)
 
     def set_outer(self, lo, hi, D):
         self.lo, self.hi, self.outer_deg = float(lo), float(hi), int(D)
@@ -170,6 +178,9 @@ class _PlanBase:
             L.load().adrt_plan_destroy(self._plan)
             self._plan = None
             self._keep = []  # releases this plan's hold on the shared dense tables
+        for a in ("_dimg", "_dimg2", "_dsino"):
+            if getattr(self, a, None) is not None:
+                getattr(self, a).free()
         if getattr(self, "eng", None):
             self.eng.close()
 
@@ -186,7 +197,9 @@ class FastLpHip(_PlanBase):
     KIND = L.KIND_LP
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None,
-                 power_iters=40, safety=1.5, dense_cap=None, precision="fp64", max_batch=1, outer=None, refine=0):
+                 power_iters=40, safety=1.5, dense_cap=None, precision="fp64", max_batch=1, outer=None, refine=0,
+                 ctx=None):
+        self.ctx = ctx
         self.precision = precision
         self.max_batch = max_batch
         self.refine = refine
@@ -223,7 +236,8 @@ class FastHybridHip(_PlanBase):
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None, g1_deg=14,
                  power_iters=40, safety=1.5, dense_cap=None, precond="g1", precision="fp64", max_batch=1, outer=None,
-                 refine=0):
+                 refine=0, ctx=None):
+        self.ctx = ctx
         self.precision = precision
         self.max_batch = max_batch
         self.refine = refine

@@ -57,6 +57,13 @@ SIGNATURES = {
     "adrt_plan_bytes": (ULL, [P]),
     "adrt_plan_create": (I, [ctypes.POINTER(P), P]),
     "adrt_plan_destroy": (I, [P]),
+    "adrt_ctx_create": (I, [ctypes.POINTER(P), ULL]),
+    "adrt_ctx_destroy": (I, [P]),
+    "adrt_ctx_stream": (P, [P]),
+    "adrt_ctx_usage": (I, [P, ctypes.POINTER(ULL), ctypes.POINTER(ULL)]),
+    "adrt_ctx_malloc": (I, [P, ctypes.POINTER(P), ULL]),
+    "adrt_ctx_free": (I, [P, P]),
+    "adrt_plan_create_in": (I, [P, ctypes.POINTER(P), P]),
     "adrt_plan_set_outer": (I, [P, D, D, I]),
     "adrt_plan_backproject": (I, [P, P, P, I, P]),
     "adrt_plan_operator": (I, [P, P, P, I, P]),
@@ -175,6 +182,45 @@ class Stream:
             pass
 
 
+class Context:
+    """A stream and a device-memory budget (adrt_ctx_*): plans and device arrays created with ``ctx=`` charge their
+    bytes to it, and a request beyond the budget raises HipError (status -3) without allocating.  budget 0: no
+    limit."""
+
+    def __init__(self, budget=0):
+        h = P()
+        check(load().adrt_ctx_create(ctypes.byref(h), int(budget)), "ctx_create")
+        self.handle = h.value
+        self.stream = _BorrowedStream(load().adrt_ctx_stream(self.handle))
+
+    def usage(self):
+        used, budget = ULL(), ULL()
+        check(load().adrt_ctx_usage(self.handle, ctypes.byref(used), ctypes.byref(budget)), "ctx_usage")
+        return used.value, budget.value
+
+    def close(self):
+        """Fails (HipError, status -4) while plans or arrays of the context are alive."""
+        if getattr(self, "handle", None):
+            check(load().adrt_ctx_destroy(self.handle), "ctx_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _BorrowedStream(Stream):
+    """A stream owned by someone else (a context): usable like Stream, never destroyed from Python."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def close(self):
+        self.handle = None
+
+
 class PinnedArray:
     """Page-locked host buffer (adrt_host_malloc) seen as a numpy array ``a``: host<->device copies from or into it
     run asynchronously on their stream."""
@@ -203,14 +249,18 @@ class PinnedArray:
 
 
 class DeviceArray:
-    """hipMalloc'ed buffer with a numpy shape/dtype."""
+    """hipMalloc'ed buffer with a numpy shape/dtype; with ``ctx`` its bytes are charged to that Context."""
 
-    def __init__(self, shape, dtype=np.float64):
+    def __init__(self, shape, dtype=np.float64, ctx=None):
         self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._ctx = ctx
         h = P()
-        check(load().adrt_malloc(ctypes.byref(h), max(1, self.nbytes)), "malloc")
+        if ctx is None:
+            check(load().adrt_malloc(ctypes.byref(h), max(1, self.nbytes)), "malloc")
+        else:
+            check(load().adrt_ctx_malloc(ctx.handle, ctypes.byref(h), max(1, self.nbytes)), "ctx_malloc")
         self.ptr = h.value
 
     @classmethod
@@ -239,7 +289,10 @@ class DeviceArray:
 
     def free(self):
         if getattr(self, "ptr", None):
-            load().adrt_free(self.ptr)
+            if self._ctx is None:
+                load().adrt_free(self.ptr)
+            else:
+                load().adrt_ctx_free(self._ctx.handle, self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -149,6 +149,21 @@ typedef struct adrt_plan adrt_plan;
 unsigned long long adrt_plan_bytes(const adrt_plan_desc* desc); /* device bytes a plan holds (admission control) */
 int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* desc);
 int adrt_plan_destroy(adrt_plan* plan);
+
+/* Contexts: a stream and a device-memory budget for one caller (a serving loop, a worker thread).  Plans created in
+ * a context charge adrt_plan_bytes(desc) to its budget and return it when destroyed; adrt_ctx_malloc / adrt_ctx_free
+ * do the same for buffers.  A request beyond the budget fails with -3 and allocates nothing.  budget_bytes 0 means no
+ * limit.  adrt_ctx_destroy fails with -4 while plans or buffers of the context are alive.  One host thread at a time
+ * per context; contexts are independent.  (The reference has no device state; this is the admission control of the
+ * device-resident API.) */
+typedef struct adrt_ctx adrt_ctx;
+int adrt_ctx_create(adrt_ctx** out, unsigned long long budget_bytes);
+int adrt_ctx_destroy(adrt_ctx* ctx);
+void* adrt_ctx_stream(adrt_ctx* ctx); /* the context's stream (non-blocking); null for a null context */
+int adrt_ctx_usage(adrt_ctx* ctx, unsigned long long* used_bytes, unsigned long long* budget_bytes);
+int adrt_ctx_malloc(adrt_ctx* ctx, void** ptr, unsigned long long bytes);
+int adrt_ctx_free(adrt_ctx* ctx, void* ptr);
+int adrt_plan_create_in(adrt_ctx* ctx, adrt_plan** out, const adrt_plan_desc* desc);
 /* replaces the outer interval and degree (set after the setup's eigenvalue estimate) */
 int adrt_plan_set_outer(adrt_plan* plan, double lo, double hi, int outer_deg);
 /* y = backproject(b) (fastops.py:283-285 / 333-335) */

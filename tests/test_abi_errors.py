@@ -105,3 +105,18 @@ def test_rows_config_classes(L):
     finally:
         lib.adrt_rows_config_set(8, 0, old0)
         lib.adrt_rows_config_set(8, 2, old2 if old2 != old0 else 0)
+
+
+def test_ctx_calls_reject(L):
+    """Context calls validate their arguments before touching the device."""
+    lib = L.load()
+    assert lib.adrt_ctx_create(None, 0) == -1 and "null" in err(L)
+    assert lib.adrt_ctx_destroy(None) == 0
+    assert lib.adrt_ctx_stream(None) is None
+    used = ctypes.c_ulonglong()
+    assert lib.adrt_ctx_usage(None, ctypes.byref(used), None) == -1 and "null context" in err(L)
+    h = L.P()
+    assert lib.adrt_ctx_malloc(None, ctypes.byref(h), 16) == -1 and "null" in err(L)
+    assert lib.adrt_ctx_free(None, None) == -1 and "null context" in err(L)
+    d, keep = desc(L)
+    assert lib.adrt_plan_create_in(None, ctypes.byref(h), ctypes.byref(d)) == -1 and "null context" in err(L)

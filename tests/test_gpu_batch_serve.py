@@ -61,3 +61,38 @@ def test_server_results_same_alone_and_in_mix(gpu):
     st = S.latency_stats(done)
     assert st["count"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
     srv.close()
+
+
+def test_context_budget_and_stream(gpu):
+    """A plan created in a context charges its bytes to the budget, fails with -3 beyond it (allocating nothing), keeps
+    the context alive (-4 on destroy) and gives the same bits as a plan outside any context, also on the context's
+    stream."""
+    L = pkg("lib")
+    I = pkg("inverses")
+    C = pkg("plancache")
+    n = 6
+    lam, outer = C.hybrid_setup(n, dense_cap=8)
+    kw = dict(lam_max=lam, outer=outer, dense_cap=8, precision="fp32", refine=1, max_batch=2)
+    b = np.stack(_sinos([n, n], seed0=700))
+    ref = I.FastHybridHip(n, **kw)
+    want = ref.apply_batch(b)
+    tiny = L.Context(budget=4096)
+    with pytest.raises(L.HipError, match=r"\(-3\)"):
+        I.FastHybridHip(n, ctx=tiny, **kw)
+    assert tiny.usage() == (0, 4096)
+    tiny.close()
+    ctx = L.Context(budget=ref.plan_bytes + (8 << 20))
+    H = I.FastHybridHip(n, ctx=ctx, **kw)
+    used, budget = ctx.usage()
+    assert used == H.plan_bytes + H._dimg.nbytes + H._dimg2.nbytes + H._dsino.nbytes and used <= budget
+    with pytest.raises(L.HipError, match=r"\(-4\)"):
+        ctx.close()
+    with pytest.raises(L.HipError, match=r"\(-3\)"):  # a second plan does not fit in what is left
+        I.FastHybridHip(n, ctx=ctx, **kw)
+    assert ctx.usage()[0] == used
+    assert np.array_equal(H.apply_batch(b), want)
+    assert np.array_equal(H.apply_batch(b, stream=ctx.stream), want)
+    H.close()
+    ref.close()
+    assert ctx.usage()[0] == 0
+    ctx.close()
