@@ -81,7 +81,9 @@ def test_context_budget_and_stream(gpu):
         I.FastHybridHip(n, ctx=tiny, **kw)
     assert tiny.usage() == (0, 4096)
     tiny.close()
-    ctx = L.Context(budget=ref.plan_bytes + (8 << 20))
+    N = 1 << n
+    staging = 8 * 2 * (2 * N * N + 4 * (2 * N - 1) * N)  # two image arrays and one sinogram array, max_batch 2
+    ctx = L.Context(budget=ref.plan_bytes + staging + 4096)
     H = I.FastHybridHip(n, ctx=ctx, **kw)
     used, budget = ctx.usage()
     assert used == H.plan_bytes + H._dimg.nbytes + H._dimg2.nbytes + H._dsino.nbytes and used <= budget
