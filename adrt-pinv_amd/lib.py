@@ -80,6 +80,7 @@ SIGNATURES = {
     "adrt_work_bytes_fwd": (ULL, [I, I, I]),
     "adrt_work_bytes_T": (ULL, [I, I, I]),
     "adrt_forward": (I, [I, I, P, P, P, I, P]),
+    "adrt_layout_permute": (I, [I, P, P, P, I, P]),
     "adrt_transpose": (I, [I, I, P, P, P, I, P]),
 }
 

@@ -197,6 +197,11 @@ unsigned long long adrt_work_bytes_T(int n, int dtype, int S);
 int adrt_forward(int n, int dtype, const void* d_x, void* d_b, void* d_work, int S, void* stream);
 int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, int S, void* stream);
 
+/* Sinogram layout permutation, the device half of pkg_to_core / core_to_pkg (skimage_adrt_pkg.py:38-51): for each of S
+ * float64 sinograms of M = 4 (2N-1) N entries, out[s][i] = in[s][src_of[i]] where src_of[i] >= 0, else 0.  d_src_of:
+ * M int32 on the device (the host builds it from fingerprints of both transforms); d_out must not alias d_in. */
+int adrt_layout_permute(int n, const int* d_src_of, const double* d_in, double* d_out, int S, void* stream);
+
 /* Least squares min ||R x - b||_2 by CGLS on the GPU: pinv.cgls (pinv.py:23-48), the cross-check of the inverses,
  * batched over S sinograms that each stop on their own test ||R^T r_j|| <= tol ||R^T r_0||.
  * d_b: (S, 4, 2N-1, N) and d_x: (S, N, N) float64 on the device; x0 = 1 starts from d_x, 0 from zero.  iters (host,

@@ -18,4 +18,6 @@ Parity pins (what each module is checked against, see ``tests/test_oracle.py``):
   inverses - L R x = x invariant, G1 spectrum / dense closed form, packing = per-quadrant operators,
              golden end-to-end fixtures of FastLp / FastHybrid at fixed setup scalars.
   lsq      - dense pseudoinverse for n <= 4.
+  layout   - a stand-in package layout that is a known permutation of the core's is recovered exactly (the adrt
+             package is not installed, so its own layout is unpinned).
 """

@@ -3,7 +3,8 @@
 import numpy as np
 import pytest
 
-from conftest import pkg
+from conftest import other_package, pkg, relmax
+from oracle import layout as olay
 from oracle import radon as orad
 
 pytestmark = pytest.mark.gpu
@@ -74,3 +75,44 @@ def test_batched_forward_matches_single(gpu):
     out = R.adrt(X)
     for s in range(3):
         assert np.array_equal(out[s], orad.adrt(n, X[s], dtype=np.int64))
+
+
+@pytest.mark.parametrize("n", [1, 4, 8])
+def test_layout_permute_matches_reference_maps(gpu, n):
+    """The device permutation, both directions, on batches of 5 and 1 sinograms, equals the reference's placement
+    with the reference's maps (skimage_adrt_pkg.py:17-51) bit for bit; the fingerprints use the GPU transform."""
+    LY = pkg("layout")
+    L = pkg("lib")
+    N = 1 << n
+    core = lambda x: orad.adrt(n, x)
+    pk = other_package(n, core)
+    print(f"layout n={n}", flush=True)
+    lm = LY.LayoutMap(n, pk)
+    maps = olay.layout_maps(n, core, pk)
+    rng = np.random.default_rng(20 + n)
+    bp = rng.standard_normal((5, 4, 2 * N - 1, N))  # noisy data: every entry set, structural ones included
+    want = np.stack([olay.pkg_to_core(b, n, maps) for b in bp])
+    assert np.array_equal(lm.pkg_to_core(bp), want)
+    assert np.array_equal(lm.pkg_to_core(bp[2]), want[2])
+    bc = rng.standard_normal((5, 4, 2 * N - 1, N))
+    assert np.array_equal(lm.core_to_pkg(bc), np.stack([olay.core_to_pkg(b, n, maps) for b in bc]))
+    d = L.DeviceArray((1,), np.float64)
+    assert L.load().adrt_layout_permute(n, lm._to_core.ptr, d.ptr, lm._to_pkg.ptr, 0, None) == 0  # S = 0: nothing
+    assert L.load().adrt_layout_permute(n, lm._to_core.ptr, d.ptr, d.ptr, 1, None) == -1  # aliasing refused
+
+
+def test_inverse_of_package_layout_data(gpu):
+    """Data in the other layout, moved to the core's on the GPU, inverts like the core's own data."""
+    LY = pkg("layout")
+    I = pkg("inverses")
+    C = pkg("plancache")
+    R = pkg("radon")
+    n, N = 6, 64
+    pk = other_package(n, lambda x: R.adrt(x))
+    lm = LY.LayoutMap(n, pk)
+    x = pkg("phantom").ellipses(31, N)
+    lam, outer = C.hybrid_setup(n, dense_cap=8)
+    H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=8)
+    xp = H.apply(lm.pkg_to_core(pk(x)))
+    assert relmax(xp, H.apply(R.adrt(x))) < 1e-12
+    assert np.linalg.norm(xp - x) / np.linalg.norm(x) < 1e-6

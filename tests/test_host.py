@@ -4,8 +4,10 @@
 import numpy as np
 import pytest
 
-from conftest import golden, pkg
+from conftest import golden, other_package, pkg
 from oracle import inverses as oiv
+from oracle import layout as olay
+from oracle import radon as orad
 
 
 def test_degrees_and_sym_match_oracle():
@@ -156,3 +158,28 @@ def test_gram_and_lstsq_cheb_validation():
         lsq.lstsq_cheb(np.zeros((4, 15, 8)), lam_min=10.0, lam_max=5.0)
     with pytest.raises(ValueError):
         lsq.lstsq_cheb(np.zeros((4, 8, 8)), lam_min=1.0, lam_max=5.0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 6])
+def test_layout_maps_pair_like_the_reference(n):
+    """Exact integer fingerprints give the pairs of the reference's rounded-fingerprint grouping
+    (skimage_adrt_pkg.py:17-35), duplicates included (both pair a group's entries in ascending index order)."""
+    LY = pkg("layout")
+    core = lambda x: orad.adrt(n, x)
+    pk = other_package(n, core)
+    ci, pi = LY.layout_maps(n, pk, core)
+    rci, rpi = olay.layout_maps(n, core, pk)
+    assert len(ci) == len(rci)
+    assert set(zip(ci.tolist(), pi.tolist())) == set(zip(rci.tolist(), rpi.tolist()))
+
+
+def test_layout_maps_reject_foreign_transforms():
+    LY = pkg("layout")
+    n, N = 3, 8
+    core = lambda x: orad.adrt(n, x)
+    with pytest.raises(ValueError, match="same lines"):
+        LY.layout_maps(n, lambda x: 2.0 * core(x), core)
+    with pytest.raises(ValueError, match="entries"):
+        LY.layout_maps(n, lambda x: core(x)[:, :N], core)
+    with pytest.raises(ImportError, match="pkg_adrt"):  # the adrt package is not installed here
+        LY.layout_maps(n, None, core)

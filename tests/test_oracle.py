@@ -7,8 +7,9 @@ transposed operand) fails it; none of them retypes the formula under test.
 import numpy as np
 import pytest
 
-from conftest import golden, relmax
+from conftest import golden, other_layout, other_package, relmax
 from oracle import inverses as oiv
+from oracle import layout as olay
 from oracle import lsq, radon, refcpp
 from oracle import rows as orw
 
@@ -327,3 +328,28 @@ def test_gram_dense_and_power_bound(n):
     D = lsq.cheb_degree(lo, hi, tol)
     rho = (np.sqrt(hi / lo) - 1) / (np.sqrt(hi / lo) + 1)
     assert 2 * rho**D <= tol < 2 * rho ** (D - 1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_layout_maps_recover_a_known_permutation(n):
+    """The reference's fingerprint maps (skimage_adrt_pkg.py:17-51), given a package whose layout is a known
+    permutation of the core's, move fresh transforms of new images between the layouts exactly and pair every entry
+    that a line reaches once."""
+    N = 1 << n
+    core = lambda x: radon.adrt(n, x)
+    pk = other_package(n, core)
+    maps = olay.layout_maps(n, core, pk)
+    ci, pi = maps
+    # integer images: every line sum is exact, so entries that sum one line along different quadrant paths are equal
+    # and the maps (which pair such duplicates by index, not by position) must move the data exactly
+    x = np.random.default_rng(11).integers(-1000, 1000, (N, N)).astype(float)
+    bc, bp = core(x), pk(x)
+    assert np.array_equal(olay.pkg_to_core(bp, n, maps), bc)
+    # the package's entries are a permutation of the core's, so core_to_pkg restores every entry, zeros included
+    assert np.array_equal(olay.core_to_pkg(bc, n, maps), bp)
+    # float images: duplicates differ only by the order of their additions
+    x = np.random.default_rng(12).standard_normal((N, N))
+    assert relmax(olay.pkg_to_core(pk(x), n, maps), core(x)) < 1e-14
+    reached = np.flatnonzero(core(np.ones((N, N))).ravel())
+    assert np.array_equal(np.sort(ci), reached) and len(set(pi.tolist())) == len(pi)
+    assert np.array_equal(np.sort(pi), np.sort(np.argsort(other_layout(n))[reached]))
