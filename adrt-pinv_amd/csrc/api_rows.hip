@@ -286,6 +286,21 @@ int adrt_stream_create(void** stream) {
     return OK;
 }
 
+int adrt_stream_create_own_queue(void** stream) {
+    ADRT_REQUIRE(stream, "adrt_stream_create_own_queue: null pointer");
+    int dev = 0, cus = 0;
+    ADRT_HIP(hipGetDevice(&dev));
+    ADRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ADRT_REQUIRE(cus > 0 && cus <= 1024, "adrt_stream_create_own_queue: unexpected CU count");
+    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+    for (int c = 0; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+    hipStream_t s;
+    // a stream with a CU mask (here: every CU) is given a hardware queue of its own instead of one from the shared pool
+    ADRT_HIP(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+    *stream = (void*)s;
+    return OK;
+}
+
 int adrt_stream_destroy(void* stream) {
     if (stream) ADRT_HIP(hipStreamDestroy((hipStream_t)stream));
     return OK;

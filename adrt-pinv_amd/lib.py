@@ -37,6 +37,7 @@ SIGNATURES = {
     "adrt_host_malloc": (I, [ctypes.POINTER(P), ULL]),
     "adrt_host_free": (I, [P]),
     "adrt_stream_create": (I, [ctypes.POINTER(P)]),
+    "adrt_stream_create_own_queue": (I, [ctypes.POINTER(P)]),
     "adrt_stream_destroy": (I, [P]),
     "adrt_stream_sync": (I, [P]),
     "adrt_stream_wait": (I, [P, P]),
@@ -158,11 +159,16 @@ def gpu_available():
 
 
 class Stream:
-    """A non-blocking HIP stream owned by Python."""
+    """A HIP stream owned by Python: non-blocking, or with own_queue on a hardware queue of its own
+    (adrt_stream_create_own_queue), so its work never waits behind another stream's (it synchronises with the null
+    stream)."""
 
-    def __init__(self):
+    def __init__(self, own_queue=False):
         h = P()
-        check(load().adrt_stream_create(ctypes.byref(h)), "stream_create")
+        if own_queue:
+            check(load().adrt_stream_create_own_queue(ctypes.byref(h)), "stream_create_own_queue")
+        else:
+            check(load().adrt_stream_create(ctypes.byref(h)), "stream_create")
         self.handle = h.value
 
     def sync(self):

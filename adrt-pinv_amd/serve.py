@@ -37,7 +37,7 @@ class GpuSlot:
     page-locked host buffers on the slot's stream; the inverse itself is replayed from a hipGraph captured once per
     batch size, so a launch costs three enqueues and the completion is one event."""
 
-    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=4, setup=None, **kw):
+    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=4, setup=None, stream=None, **kw):
         from . import lib as L
         from .inverses import FastHybridHip
 
@@ -47,7 +47,7 @@ class GpuSlot:
         lam, outer = setup
         self.plan = FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=cap, precision=precision,
                                   max_batch=max_batch, refine=refine, **kw)
-        self.stream = L.Stream()
+        self.stream = stream or L.Stream()  # a stream shared with other slots runs their launches in turn
         self.d_b = L.DeviceArray((max_batch, 4, 2 * N - 1, N))
         self.d_x = L.DeviceArray((max_batch, N, N))
         self.event = L.Event()
@@ -112,15 +112,34 @@ class GpuSlot:
 
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
-                 precision="fp32", refine=1, tune=True, **kw):
+                 precision="fp32", refine=1, tune=True, stream_per_size=True, streams_per_size=1, **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
-        deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory)."""
+        deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory).
+        stream_per_size: the slots of one N share one stream on a hardware queue of its own, so a long batch of one
+        size never holds up another size's launches (ordinary streams share a few hardware queues, and a small-N
+        batch could wait behind an N=1024 one); the slots of one size then run their batches in turn, on
+        ``streams_per_size`` such streams (dealt to the slots of the size in turn)."""
         self.keys = list(keys)
         self.pending = {k: deque() for k in self.keys}
         self.slots = []
         self.rejected = []
         self.done = []
-        factory = slot_factory or (lambda n, cap, mb: GpuSlot(n, cap, precision, refine, mb, setups[(n, cap)], **kw))
+        self._streams = {}
+        made = {}
+
+        def gpu_slot(n, cap, mb):
+            st = None
+            if stream_per_size:
+                from . import lib as L
+
+                j = made.get(n, 0) % max(1, streams_per_size)
+                made[n] = made.get(n, 0) + 1
+                if (n, j) not in self._streams:
+                    self._streams[(n, j)] = L.Stream(own_queue=True)
+                st = self._streams[(n, j)]
+            return GpuSlot(n, cap, precision, refine, mb, setups[(n, cap)], stream=st, **kw)
+
+        factory = slot_factory or gpu_slot
         if mem_budget is None and slot_factory is None:
             from . import lib as L
 

@@ -64,6 +64,12 @@ int adrt_host_malloc(void** ptr, unsigned long long bytes);
 int adrt_host_free(void* ptr);
 int adrt_memset(void* dst, int value, unsigned long long bytes, void* stream);
 int adrt_stream_create(void** stream);
+/* A stream on a hardware queue of its own.  HIP maps ordinary streams onto a small pool of hardware queues (4 by
+ * default), so work queued on one stream can hold up another stream that shares its queue; a stream created with a CU
+ * mask (here: every CU) gets a queue of its own (measured on MI355X).  For long-lived streams whose work must not wait
+ * behind other streams', e.g. one per request size in a server.  Synchronises with the null stream like
+ * hipStreamCreate's streams. */
+int adrt_stream_create_own_queue(void** stream);
 int adrt_stream_destroy(void* stream);
 int adrt_stream_sync(void* stream);
 int adrt_stream_wait(void* waiting, void* signal); /* `waiting` waits for the work enqueued so far on `signal` */
