@@ -111,7 +111,9 @@ struct G1Cfg {
     static constexpr int RB = R / 2;           // 2 x 2 cells per region side
     static constexpr int TB = TILE / 2;        // cells per tile side
     static constexpr int HB = HALO / 2;        // halo rings (cells)
-    static constexpr int CP = RB + 2;          // LDS pitch in cells: the region plus a zero border cell on each side
+    // LDS pitch in cells: the region plus a zero border cell on each side, made odd so that a column of cells (the
+    // halo rings' left and right sides, read two lanes per row) spreads over all banks instead of a few
+    static constexpr int CP = (RB + 2) | 1;
     static constexpr int BUF = CP * CP;
     static constexpr int NB = RB * RB;
     static constexpr int BPT = (NB + G1_THREADS - 1) / G1_THREADS;

@@ -1,7 +1,7 @@
 """Benchmark: hybrid ADRT inverses per second at N = 256 on one MI355X, plus mixed-N batching and serving.
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 8] [--dense-cap 16] [--no-cpu] [--no-serve]
-    python bench.py --rows-only      # short workload for rocprofv3 --pmc runs of the row kernel
+    python bench.py --rows-only [--mapping LOGE,DENSE]   # short workload for rocprofv3 --pmc runs of the row kernel
     python bench.py --large          # adds the N=2048, batch-16 FP32 vs FP64 leg
 
 A step is one batched hybrid inverse (FP32 row engine with TwoSum-compensated Chebyshev accumulation; FP64 dense
@@ -110,19 +110,21 @@ def make_sinos(n, count, seed0, noise_seed0, level=0.01):
     return np.stack([ph.add_noise(B0[i], level, noise_seed0 + i) for i in range(count)])
 
 
-def time_rows_kernel(L, H, S, st, reps=10):
-    """Average duration of the row kernel of the G_h pass: events recorded immediately before and after it."""
-    ev0, ev1 = L.Event(), L.Event()
-    L.check(L.load().adrt_plan_time_rows(H._plan, ev0.handle, ev1.handle), "plan_time_rows")
+def time_rows_kernel(L, H, S, st, reps=20):
+    """Average duration of the row kernel of the G_h pass: a pair of events recorded immediately before and after each
+    of ``reps`` launches, enqueued back to back as in a pass (no host wait between them, so the clocks and caches are
+    those of a running inverse), read after one synchronisation."""
+    lib = L.load()
+    evs = [(L.Event(), L.Event()) for _ in range(reps)]
     try:
-        L.check(L.load().adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")  # warm-up
-        ts = []
-        for _ in range(reps):
-            L.check(L.load().adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")
-            st.sync()
-            ts.append(ev0.elapsed_ms(ev1))
+        L.check(lib.adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")  # warm-up
+        for ev0, ev1 in evs:
+            L.check(lib.adrt_plan_time_rows(H._plan, ev0.handle, ev1.handle), "plan_time_rows")
+            L.check(lib.adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")
     finally:
-        L.check(L.load().adrt_plan_time_rows(H._plan, None, None), "plan_time_rows")
+        L.check(lib.adrt_plan_time_rows(H._plan, None, None), "plan_time_rows")
+    st.sync()
+    ts = [ev0.elapsed_ms(ev1) for ev0, ev1 in evs]
     return float(np.mean(ts)), ts
 
 
@@ -238,7 +240,14 @@ def rows_only(a):
     H64 = I.FastHybridHip(n, dense_cap=cap, precision="fp64", max_batch=1)
     H32 = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B,
                           outer=(H64.lo, H64.hi, H64.outer_deg), refine=a.refine)
-    pkg("autotune").tune(H32, B, persist=False)
+    if a.mapping:
+        # counter runs serialise and slow the kernels, so the tuner's timings there are not the real ones: take the
+        # mapping the timed run chose
+        loge, dense = (int(v) for v in a.mapping.split(","))
+        L.check(L.load().adrt_rows_config_set(n, 1, loge), "rows_config_set")
+        L.check(L.load().adrt_dense_config_set(n, 1, dense), "dense_config_set")
+    else:
+        pkg("autotune").tune(H32, B, persist=False)
     st = L.Stream()
     d_x = L.DeviceArray.from_host(np.random.default_rng(0).standard_normal((B, N, N)), st)
     d_y = L.DeviceArray((B, N, N))
@@ -274,6 +283,8 @@ def main():
     ap.add_argument("--serve-rates", default="200,400", help="Poisson arrival rates (req/s), comma separated")
     ap.add_argument("--serve-duration", type=float, default=2.0)
     ap.add_argument("--rows-only", action="store_true")
+    ap.add_argument("--mapping", default=None,
+                    help="with --rows-only: 'loge,dense' (row mapping and dense placement) instead of autotuning")
     ap.add_argument("--large", action="store_true",
                     help="also the N=2048 batch-16 FP32 vs FP64 leg (about 45 s of setup and 50 GiB of dense tables)")
     a = ap.parse_args()
