@@ -9,7 +9,7 @@ table GEMM, preconditioner, outer iteration and the last refinement steps) of B 
 are resident in HBM, replayed from one hipGraph.  ``value`` is inverses/s (B * K / elapsed).  Beside it the line
 reports the plain FP64 GPU path (one sinogram per call, no graph, no batching), the relative error of the FP32 result
 against the FP64 one, the roofline of the dominant kernel (the register row kernel of the G_h pass, timed with HIP
-events recorded around it on its own stream), the reference C++ row engine timed on the host cores, a 512-sinogram
+events around it in 20 back-to-back G_h passes on its stream), the reference C++ row engine timed on the host cores, a 512-sinogram
 uneven batch in one graph launch and a Poisson serving trace over mixed N with p50 / p99 latency.
 """
 
@@ -110,17 +110,19 @@ def make_sinos(n, count, seed0, noise_seed0, level=0.01):
     return np.stack([ph.add_noise(B0[i], level, noise_seed0 + i) for i in range(count)])
 
 
-def time_rows_kernel(L, H, S, st, reps=20):
-    """Average duration of the row kernel of the G_h pass: a pair of events recorded immediately before and after each
-    of ``reps`` launches, enqueued back to back as in a pass (no host wait between them, so the clocks and caches are
-    those of a running inverse), read after one synchronisation."""
+def time_rows_kernel(L, H, d_x, d_y, S, st, reps=20):
+    """Average duration of the row kernel of the G_h pass as it runs in an inverse: ``reps`` whole G_h passes
+    (offset FFT, row kernel, inverse FFT, combine) enqueued back to back on the stream, with a pair of events recorded
+    immediately before and after the row kernel of each (adrt_plan_time_rows), read after one synchronisation.  (Row
+    kernels launched back to back with nothing between them run some 10% slower than inside a pass: the chip's power
+    limit then lowers the clock.)"""
     lib = L.load()
     evs = [(L.Event(), L.Event()) for _ in range(reps)]
+    H.operator_device(d_x, d_y, S, st)  # warm-up
     try:
-        L.check(lib.adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")  # warm-up
         for ev0, ev1 in evs:
             L.check(lib.adrt_plan_time_rows(H._plan, ev0.handle, ev1.handle), "plan_time_rows")
-            L.check(lib.adrt_plan_rows_pass(H._plan, 1, S, st.handle), "rows_pass")
+            H.operator_device(d_x, d_y, S, st)
     finally:
         L.check(lib.adrt_plan_time_rows(H._plan, None, None), "plan_time_rows")
     st.sync()
@@ -366,11 +368,10 @@ def main():
     plain32_ms, _, rel_p = batched(H32p)
     H32p.close()
 
-    # dominant kernel: the register row kernel of the G_h pass, events around the kernel on its own stream
+    # dominant kernel: the register row kernel of the G_h pass, events around it inside whole passes
     ops = H32.ops_op()
     d_tmp = L.DeviceArray((B, N, N))
-    H32.operator_device(d_x32, d_tmp, B, st)  # leaves the rows of a real G_h pass in the plan
-    rows_ms, _ = time_rows_kernel(L, H32, B, st)
+    rows_ms, _ = time_rows_kernel(L, H32, d_x32, d_tmp, B, st)
     fl, nb = rows_kernel_work(H32, B, ops)
     dfl = dense_work(H32, B)[0]
     achieved = fl / (rows_ms * 1e-3)

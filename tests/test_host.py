@@ -183,3 +183,12 @@ def test_layout_maps_reject_foreign_transforms():
         LY.layout_maps(n, lambda x: core(x)[:, :N], core)
     with pytest.raises(ImportError, match="pkg_adrt"):  # the adrt package is not installed here
         LY.layout_maps(n, None, core)
+
+
+def test_dense_placement_tie_break():
+    """Placements within 1% of the fastest tie, and the most dense waves win; a launch of its own only outright."""
+    A = pkg("autotune")
+    assert A.pick_dense({-1: 0.304, 1: 0.277, 2: 0.277, 4: 0.281}) == 2
+    assert A.pick_dense({1: 0.300, 2: 0.280, 4: 0.2801}) == 4
+    assert A.pick_dense({-1: 0.270, 1: 0.272, 2: 0.2725}) == -1
+    assert A.pick_dense({-1: 0.300, 1: 0.250, 2: 0.260, 4: 0.300}) == 1
