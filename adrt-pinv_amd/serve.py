@@ -224,6 +224,10 @@ class Server:
     def close(self):
         for s in self.slots:
             s.close()
+        for st in self._streams.values():
+            st.sync()
+            st.close()
+        self._streams.clear()
 
 
 def poisson_trace(rate, duration, sizes, weights, caps, pool, seed=7):

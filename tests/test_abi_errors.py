@@ -120,3 +120,7 @@ def test_ctx_calls_reject(L):
     assert lib.adrt_ctx_free(None, None) == -1 and "null context" in err(L)
     d, keep = desc(L)
     assert lib.adrt_plan_create_in(None, ctypes.byref(h), ctypes.byref(d)) == -1 and "null context" in err(L)
+
+
+def test_own_queue_stream_rejects_null(L):
+    assert L.load().adrt_stream_create_own_queue(None) == -1 and "null" in err(L)
