@@ -11,9 +11,9 @@ With the mapping fixed it then places the dense low-frequency GEMM of the pass (
 inside the row kernel's launch, at most 1, 2 or 4 per SIMD, or as a launch of its own after the rows
 (adrt_dense_config_set).  Few dense waves leave the vector ALUs to the rows; more hide the GEMM's load latency better
 when the dense work is large (N >= 1024, where a third of the frequencies can be dense).  Placements within 1% of
-the fastest count as a tie, won by the most dense waves: each worker then takes fewer tiles, so fewer of its
-frequency's rows have left L2 when it reads them again (at N=256, 64 sinograms: 1.30 against 1.58 times the
-algorithmic HBM bytes, at the same time).
+the fastest count as a tie, won by two dense waves per SIMD, then one, then four: with two each worker takes half
+the tiles it would take with one, so fewer of its frequency's rows have left L2 when it reads them again (at N=256,
+64 sinograms: 1.30 against 1.58 times the algorithmic HBM bytes, at the same time).
 
 An FP32 plan with FP64 refinement passes (refine >= 1) runs those passes at the FP32 plan's batch depth, where the
 best FP64 mapping can differ from a plain FP64 plan's: the tuner times them too, on the rows an apply's last (FP64)
@@ -32,12 +32,19 @@ DENSE_CANDIDATES = (-1, 1, 2, 4)
 DENSE_TIE = 0.01
 
 
+DENSE_TIE_ORDER = (2, 1, 4, 8)
+
+
 def pick_dense(times):
-    """The dense placement of ``times`` ({mode: ms}): the fastest, or among those within DENSE_TIE of it the one with
-    the most dense waves per SIMD (a launch of its own, -1, only when it is the fastest outright)."""
+    """The dense placement of ``times`` ({mode: ms}): the fastest, or among those within DENSE_TIE of it the first of
+    DENSE_TIE_ORDER (a launch of its own, -1, only when it is the fastest outright).  Two waves per SIMD give each
+    worker half the tiles of one, so fewer of a frequency's rows are re-read from HBM; four take wave slots from the
+    rows that these sizes' dense work does not need."""
     best = min(times, key=times.get)
+    if best == -1:
+        return best
     near = [m for m, t in times.items() if m > 0 and t <= times[best] * (1 + DENSE_TIE)]
-    return max(near) if near and best != -1 else best
+    return min(near, key=lambda m: DENSE_TIE_ORDER.index(m) if m in DENSE_TIE_ORDER else len(DENSE_TIE_ORDER))
 
 
 class TuneResult(tuple):
