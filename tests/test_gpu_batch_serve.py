@@ -98,3 +98,42 @@ def test_context_budget_and_stream(gpu):
     ref.close()
     assert ctx.usage()[0] == 0
     ctx.close()
+
+
+def test_independent_contexts_run_concurrently(gpu):
+    """Two host threads, each with its own context, plan and stream, apply at the same time and get the bits of the
+    same plans run one after the other."""
+    import threading
+
+    L = pkg("lib")
+    I = pkg("inverses")
+    C = pkg("plancache")
+    n = 6
+    lam, outer = C.hybrid_setup(n, dense_cap=8)
+    kw = dict(lam_max=lam, outer=outer, dense_cap=8, precision="fp32", refine=1, max_batch=2)
+    bs = [np.stack(_sinos([n, n], seed0=800 + 10 * t)) for t in range(2)]
+    ctxs = [L.Context() for _ in range(2)]
+    plans = [I.FastHybridHip(n, ctx=c, **kw) for c in ctxs]
+    want = [plans[t].apply_batch(bs[t], stream=ctxs[t].stream) for t in range(2)]
+    got = [None, None]
+    errs = []
+
+    def work(t):
+        try:
+            for _ in range(3):
+                got[t] = plans[t].apply_batch(bs[t], stream=ctxs[t].stream)
+        except Exception as e:  # reported below, in the main thread
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=120)
+    assert not errs and all(not x.is_alive() for x in th)
+    for t in range(2):
+        assert np.array_equal(got[t], want[t])
+    for p in plans:
+        p.close()
+    for c in ctxs:
+        c.close()
