@@ -116,3 +116,24 @@ def test_inverse_of_package_layout_data(gpu):
     xp = H.apply(lm.pkg_to_core(pk(x)))
     assert relmax(xp, H.apply(R.adrt(x))) < 1e-12
     assert np.linalg.norm(xp - x) / np.linalg.norm(x) < 1e-6
+
+
+def test_layout_permute_round_trip_at_n2048(gpu):
+    """At the largest serving size (N=2048, 33.5M entries per sinogram) the device permutation matches the maps'
+    placement and core_to_pkg undoes pkg_to_core on every entry a line reaches."""
+    LY = pkg("layout")
+    R = pkg("radon")
+    n, N = 11, 2048
+    print(f"layout round trip n={n}", flush=True)
+    lm = LY.LayoutMap(n, other_package(n, lambda x: R.adrt(x)))
+    rng = np.random.default_rng(5)
+    bp = rng.standard_normal((2, 4, 2 * N - 1, N))
+    bc = lm.pkg_to_core(bp)
+    want = np.zeros((2, lm.M))
+    want[:, lm.core_idx] = bp.reshape(2, -1)[:, lm.pkg_idx]
+    assert np.array_equal(bc.reshape(2, -1), want)
+    back = lm.core_to_pkg(bc).reshape(2, -1)
+    reached = np.zeros(lm.M, bool)
+    reached[lm.pkg_idx] = True
+    assert np.array_equal(back[:, reached], bp.reshape(2, -1)[:, reached])
+    assert not back[:, ~reached].any()
