@@ -205,7 +205,8 @@ int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, i
 
 /* Sinogram layout permutation, the device half of pkg_to_core / core_to_pkg (skimage_adrt_pkg.py:38-51): for each of S
  * float64 sinograms of M = 4 (2N-1) N entries, out[s][i] = in[s][src_of[i]] where src_of[i] >= 0, else 0.  d_src_of:
- * M int32 on the device (the host builds it from fingerprints of both transforms); d_out must not alias d_in. */
+ * M int32 on the device, each in [-1, M) (the kernel does not check; the host builds the map from fingerprints of
+ * both transforms); d_out must not alias d_in. */
 int adrt_layout_permute(int n, const int* d_src_of, const double* d_in, double* d_out, int S, void* stream);
 
 /* Least squares min ||R x - b||_2 by CGLS on the GPU: pinv.cgls (pinv.py:23-48), the cross-check of the inverses,
