@@ -52,7 +52,8 @@ int adrt_cert_rowsum(int n, int K, const double* omega, int m_lo, double* out);
 
 /* ------------------------------------------------------------------ layer 2: device-resident API */
 
-const char* adrt_last_error(void); /* message of the last failure on this thread */
+/* message of the last failure on this thread; the pointer stays valid until the next failing call on the thread */
+const char* adrt_last_error(void);
 int adrt_device_count(void);
 int adrt_malloc(void** ptr, unsigned long long bytes);
 int adrt_free(void* ptr);

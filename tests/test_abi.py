@@ -60,3 +60,50 @@ def test_every_package_module_imports():
         if m in ("__init__", "build"):
             continue
         pkg(m)
+
+
+C_CALLER = r"""
+#include <stdio.h>
+#include "adrt_hip.h"
+int main(void) {
+    int deg[128];
+    double ac[128];
+    adrt_plan_desc d = {0};
+    adrt_ctx* ctx = 0;
+    for (int i = 0; i < 128; ++i) { deg[i] = 1 + i % 5; ac[i] = 1.0; }
+    d.kind = ADRT_KIND_HYBRID; d.n = 6; d.precision = ADRT_PREC_F32; d.precond = ADRT_PRECOND_G1;
+    d.g1_deg = 14; d.outer_deg = 10; d.lo = 1.0; d.hi = 2.0; d.tau = 12.8;
+    d.deg = deg; d.a = ac; d.c = ac; d.max_batch = 4;
+    printf("%llu\n", adrt_plan_bytes(&d));
+    int st = adrt_ctx_create(0, 0);
+    printf("%d %s\n", st, adrt_last_error());
+    printf("%d\n", adrt_ctx_destroy(ctx));
+    return 0;
+}
+"""
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """A C99 caller of include/adrt_hip.h compiles with gcc, links the library and gets the same host-side answers as
+    the ctypes binding (adrt_plan_bytes; argument errors before any device call)."""
+    import subprocess
+
+    import numpy as np
+
+    L = pkg("lib")
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(L.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-ladrt_hip", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    d = L.PlanDesc()
+    deg = np.array([1 + i % 5 for i in range(128)], dtype=np.int32)
+    ac = np.ones(128)
+    d.kind, d.n, d.precision, d.precond = L.KIND_HYBRID, 6, L.PRECISION["fp32"], L.PRECOND["g1"]
+    d.g1_deg, d.outer_deg, d.lo, d.hi, d.tau = 14, 10, 1.0, 2.0, 12.8
+    d.deg, d.a, d.c, d.max_batch = L.ptr(deg), L.ptr(ac), L.ptr(ac), 4
+    assert int(out[0]) == L.load().adrt_plan_bytes(ctypes.byref(d)) > 0
+    assert out[1].startswith("-1 ") and "null" in out[1]
+    assert out[2] == "0"
