@@ -99,6 +99,10 @@ class LayoutMap:
         do.free()
         return out[0] if single else out
 
+    def free(self):
+        self._to_core.free()
+        self._to_pkg.free()
+
     def pkg_to_core(self, bp):
         return self._host(self.pkg_to_core_device, bp)
 
@@ -106,7 +110,14 @@ class LayoutMap:
         return self._host(self.core_to_pkg_device, bc)
 
 
-_maps = {}
+_maps = {}  # (n, pkg_adrt) -> LayoutMap; each holds 2 * 4 (2N-1) N int32 on the device (268 MB at N = 2048)
+
+
+def clear_maps():
+    """Free the device maps that pkg_to_core / core_to_pkg keep per size."""
+    for m in _maps.values():
+        m.free()
+    _maps.clear()
 
 
 def _map(n, pkg_adrt):

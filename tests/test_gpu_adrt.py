@@ -99,6 +99,11 @@ def test_layout_permute_matches_reference_maps(gpu, n):
     d = L.DeviceArray((1,), np.float64)
     assert L.load().adrt_layout_permute(n, lm._to_core.ptr, d.ptr, lm._to_pkg.ptr, 0, None) == 0  # S = 0: nothing
     assert L.load().adrt_layout_permute(n, lm._to_core.ptr, d.ptr, d.ptr, 1, None) == -1  # aliasing refused
+    # the module-level functions keep one map per size until clear_maps()
+    assert np.array_equal(LY.pkg_to_core(bp[1], n, pk), want[1])
+    assert len(LY._maps) >= 1
+    LY.clear_maps()
+    assert not LY._maps
 
 
 def test_inverse_of_package_layout_data(gpu):
