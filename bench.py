@@ -283,7 +283,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", default="200,400", help="Poisson arrival rates (req/s), comma separated")
-    ap.add_argument("--serve-duration", type=float, default=2.0)
+    ap.add_argument("--serve-duration", type=float, default=5.0,
+                    help="seconds of Poisson arrivals per rate (5 s at 200 req/s: about 1000 requests behind the p99)")
     ap.add_argument("--rows-only", action="store_true")
     ap.add_argument("--mapping", default=None,
                     help="with --rows-only: 'loge,dense' (row mapping and dense placement) instead of autotuning")
