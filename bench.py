@@ -68,14 +68,15 @@ def rows_kernel_work(H, S, ops):
 
 
 def pmc_traffic():
-    """HBM bytes per launch of the row kernel from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
-    calibrated against a copy of known size), or None."""
+    """(HBM bytes per launch of the row kernel, "loge,dense" mapping they were counted at) from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or (None, None)."""
     try:
         txt = open(PMC_SUMMARY).read()
     except OSError:
-        return None
+        return None, None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, flags=re.M)
-    return float(m.group(1)) if m else None
+    mp = re.search(r"--mapping (\S+)", txt)
+    return (float(m.group(1)) if m else None), (mp.group(1) if mp else None)
 
 
 def cpu_baseline(H, cap, b, threads, budget_s=20.0):
@@ -376,8 +377,9 @@ def main():
     fl, nb = rows_kernel_work(H32, B, ops)
     dfl = dense_work(H32, B)[0]
     achieved = fl / (rows_ms * 1e-3)
+    traffic, traffic_map = pmc_traffic()  # counted at the mapping the profile names (row loge, dense placement)
     roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK["fp32"], "traffic": pmc_traffic(),
+            "frac": achieved / PEAK["fp32"], "traffic": traffic, "traffic_mapping": traffic_map,
             "frac_nonpacked": achieved / PEAK["fp32_nonpacked"],
             "kernel": f"rows_reg_kernel<float, {n}, e, comp> (G_h pass, with the dense MFMA tiles)",
             "kernel_ms": rows_ms, "flops_per_launch": fl, "alg_bytes_per_launch": nb,
