@@ -350,15 +350,17 @@ def main():
         for _ in range(a.warmup):
             g.launch(st)
         st.sync()
-        x = d_x32.download(st)
-        rel = np.linalg.norm((x - x64).reshape(B, -1), axis=1) / np.linalg.norm(x64.reshape(B, -1), axis=1)
         t_wall = time.time()
         e0.record(st)
         for _ in range(a.steps):
             g.launch(st)
         e1.record(st)
         ms = e0.elapsed_ms(e1) / a.steps
-        return ms, (time.time() - t_wall) * 1000.0 / a.steps, rel
+        wall = (time.time() - t_wall) * 1000.0 / a.steps
+        # the result of the last timed replay (the graph recomputes it from the resident inputs every time)
+        x = d_x32.download(st)
+        rel = np.linalg.norm((x - x64).reshape(B, -1), axis=1) / np.linalg.norm(x64.reshape(B, -1), axis=1)
+        return ms, wall, rel
 
     step_ms, wall_ms, rel = batched(H32)
     ips = B * 1000.0 / step_ms
