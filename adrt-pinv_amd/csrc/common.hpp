@@ -1,6 +1,7 @@
 // Shared definitions of the ADRT HIP library: complex arithmetic, error reporting, launch checks.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <cstdio>

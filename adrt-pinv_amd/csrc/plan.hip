@@ -75,7 +75,9 @@ struct Plan {
     RowOp ops_op[MAX_OPS];
     int nop = 0;
     int comp = 0;  // compensated FP32 Chebyshev accumulation (desc precision ADRT_PREC_F32)
-    hipEvent_t ev_rows0 = nullptr, ev_rows1 = nullptr;  // bracket every row-kernel launch when set (timing)
+    // row-kernel launches stamp their start / end into these event pairs, in turn (timing; empty: off)
+    std::vector<hipEvent_t> ev_rows0, ev_rows1;
+    size_t ev_next = 0;
     bool borrowed = false;  // dFT / WopT / WbackT belong to the caller (desc dense_on_device == 2)
     Context* ctx = nullptr;        // the context the plan's bytes are charged to (adrt_plan_create_in), or null
     unsigned long long charged = 0;
@@ -97,6 +99,14 @@ void free_plan(Plan* p) {
 
 // The row ops on the plan's rows Y, written out of place to `out` (the dense GEMM's blocks read all rows of their
 // frequency while others write).
+// the next timing event pair of the plan (adrt_plan_time_rows / adrt_plan_time_rows_ring), if any
+void timing_events(Plan* p, RowsArgs& A) {
+    if (p->ev_rows0.empty()) return;
+    const size_t i = p->ev_next++ % p->ev_rows0.size();
+    A.ev_start = p->ev_rows0[i];
+    A.ev_stop = p->ev_rows1[i];
+}
+
 int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t st, void* out,
               const c128* WT = nullptr) {
     RowsArgs A{};
@@ -145,18 +155,15 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
             A.dn.SP = 2 * S;
         }
         if (A.nk > 0) {
-            if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
+            timing_events(p, A);
             ADRT_TRY(launch_rows(A, prec, st, &dense_done));
-            if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
         }
         if (dense_done) return OK;
         if (mfma) return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
     }
-    if (p->ev_rows0) ADRT_HIP(hipEventRecord(p->ev_rows0, st));
-    ADRT_TRY(launch_rows(A, prec, st));
-    if (p->ev_rows1) ADRT_HIP(hipEventRecord(p->ev_rows1, st));
-    return OK;
+    timing_events(p, A);
+    return launch_rows(A, prec, st);
 }
 
 // One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
@@ -601,8 +608,25 @@ int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end) {
     auto* p = (Plan*)plan;
     ADRT_REQUIRE(p, "plan_time_rows: null plan");
     ADRT_REQUIRE((ev_begin == nullptr) == (ev_end == nullptr), "plan_time_rows: give both events or neither");
-    p->ev_rows0 = (hipEvent_t)ev_begin;
-    p->ev_rows1 = (hipEvent_t)ev_end;
+    p->ev_rows0.clear();
+    p->ev_rows1.clear();
+    p->ev_next = 0;
+    if (ev_begin) {
+        p->ev_rows0.push_back((hipEvent_t)ev_begin);
+        p->ev_rows1.push_back((hipEvent_t)ev_end);
+    }
+    return OK;
+}
+
+int adrt_plan_time_rows_ring(adrt_plan* plan, void* const* ev_begin, void* const* ev_end, int count) {
+    auto* p = (Plan*)plan;
+    ADRT_REQUIRE(p, "plan_time_rows_ring: null plan");
+    ADRT_REQUIRE(count >= 0 && (count == 0 || (ev_begin && ev_end)), "plan_time_rows_ring: bad event arrays");
+    for (int i = 0; i < count; ++i)
+        ADRT_REQUIRE(ev_begin[i] && ev_end[i], "plan_time_rows_ring: null event");
+    p->ev_rows0.assign((hipEvent_t const*)ev_begin, (hipEvent_t const*)ev_begin + count);
+    p->ev_rows1.assign((hipEvent_t const*)ev_end, (hipEvent_t const*)ev_end + count);
+    p->ev_next = 0;
     return OK;
 }
 

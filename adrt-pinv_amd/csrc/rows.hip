@@ -256,7 +256,11 @@ int launch_one(const RowsArgs& A, hipStream_t s) {
     const size_t lds = sizeof(cplx<T>) * (A.N + (A.N >> 1) + 1) + sizeof(c128) * 16 + 16;
     long long grid = A.rows < (1LL << 20) ? A.rows : (1LL << 20);
     if (grid < 1) return OK;
-    hipLaunchKernelGGL((rows_kernel<Tg, T, NT, E>), dim3((unsigned)grid), dim3(NT), lds, s, A);
+    if (A.ev_start)
+        hipExtLaunchKernelGGL((rows_kernel<Tg, T, NT, E>), dim3((unsigned)grid), dim3(NT), (std::uint32_t)lds, s,
+                              A.ev_start, A.ev_stop, 0, A);
+    else
+        hipLaunchKernelGGL((rows_kernel<Tg, T, NT, E>), dim3((unsigned)grid), dim3(NT), lds, s, A);
     return launch_check("rows_kernel");
 }
 

@@ -38,6 +38,9 @@ struct RowsArgs {
     int skip_dense = 0;  // rows of dense frequencies are left untouched (a dense GEMM handles them)
     int loge = 0;  // register kernel elements per lane (log2); 0 = default for (n, precision), -1 = generic kernel
     int cfg_class = -1;  // rows_config class of the pass (-1: its precision; 2: FP64 refinement pass of an FP32 plan)
+    // when set, the row kernel's launch stamps its own start and end into these events (hipExtLaunchKernel: the
+    // dispatch's timestamps, not markers that may be processed while the previous kernel still runs)
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int comp = 0;  // FP32 only: compensated (TwoSum) accumulation of the Chebyshev iterate
     // Chebyshev step weights (w_j a, w_j - 1) of parameter row k at wcoef[wcoff[k] + j], 1 <= j < deg[k]
     // (cheb_table_fill); the register kernel takes op lists with P ops only when they are given

@@ -510,7 +510,11 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         set_error("rows_reg: too many blocks");
         return E_INVALID;
     }
-    hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RB);
+    if (RA.r.ev_start)
+        hipExtLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s,
+                              RA.r.ev_start, RA.r.ev_stop, 0, RB);
+    else
+        hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RB);
     return launch_check("rows_reg_kernel");
 }
 

@@ -71,6 +71,7 @@ SIGNATURES = {
     "adrt_plan_precond": (I, [P, P, P, I, P]),
     "adrt_plan_rows_pass": (I, [P, I, I, P]),
     "adrt_plan_time_rows": (I, [P, P, P]),
+    "adrt_plan_time_rows_ring": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P), I]),
     "adrt_cgls_work_bytes": (ULL, [I, I, I]),
     "adrt_cgls": (I, [I, P, P, I, D, I, I, P, P, P, I, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),

@@ -9,7 +9,7 @@ table GEMM, preconditioner, outer iteration and the last refinement steps) of B 
 are resident in HBM, replayed from one hipGraph.  ``value`` is inverses/s (B * K / elapsed).  Beside it the line
 reports the plain FP64 GPU path (one sinogram per call, no graph, no batching), the relative error of the FP32 result
 against the FP64 one, the roofline of the dominant kernel (the register row kernel of the G_h pass, timed with HIP
-events around it in 20 back-to-back G_h passes on its stream), the reference C++ row engine timed on the host cores, a 512-sinogram
+the kernel's own dispatch timestamps inside whole applies), the reference C++ row engine timed on the host cores, a 512-sinogram
 uneven batch in one graph launch and a Poisson serving trace over mixed N with p50 / p99 latency.
 """
 
@@ -111,23 +111,28 @@ def make_sinos(n, count, seed0, noise_seed0, level=0.01):
     return np.stack([ph.add_noise(B0[i], level, noise_seed0 + i) for i in range(count)])
 
 
-def time_rows_kernel(L, H, d_x, d_y, S, st, reps=20):
-    """Average duration of the row kernel of the G_h pass as it runs in an inverse: ``reps`` whole G_h passes
-    (offset FFT, row kernel, inverse FFT, combine) enqueued back to back on the stream, with a pair of events recorded
-    immediately before and after the row kernel of each (adrt_plan_time_rows), read after one synchronisation.  (Row
-    kernels launched back to back with nothing between them run some 10% slower than inside a pass: the chip's power
-    limit then lowers the clock.)"""
+def time_rows_kernel(L, H, d_b, d_x, S, st, reps=3):
+    """Average duration of the FP32 row kernel of the G_h pass where it runs: every row-kernel launch of ``reps`` whole
+    applies stamps its own start and end (adrt_plan_time_rows_ring: the dispatch's timestamps), and the average is
+    over the FP32 G_h launches, i.e. all but the first (the FP64 backprojection) and the last ``refine`` (the FP64
+    refinement steps) of each apply."""
     lib = L.load()
-    evs = [(L.Event(), L.Event()) for _ in range(reps)]
-    H.operator_device(d_x, d_y, S, st)  # warm-up
-    try:
-        for ev0, ev1 in evs:
-            L.check(lib.adrt_plan_time_rows(H._plan, ev0.handle, ev1.handle), "plan_time_rows")
-            H.operator_device(d_x, d_y, S, st)
-    finally:
-        L.check(lib.adrt_plan_time_rows(H._plan, None, None), "plan_time_rows")
+    n_launch = H.outer_deg  # one backprojection pass and outer_deg - 1 G_h passes
+    evs = [(L.Event(), L.Event()) for _ in range(n_launch)]
+    b0 = (L.P * n_launch)(*[a.handle for a, _ in evs])
+    b1 = (L.P * n_launch)(*[z.handle for _, z in evs])
+    H.apply_device(d_b, d_x, S, st, count=False)  # warm-up
     st.sync()
-    ts = [ev0.elapsed_ms(ev1) for ev0, ev1 in evs]
+    ts = []
+    try:
+        for _ in range(reps):
+            L.check(lib.adrt_plan_time_rows_ring(H._plan, b0, b1, n_launch), "plan_time_rows_ring")
+            H.apply_device(d_b, d_x, S, st, count=False)
+            st.sync()
+            t = [a.elapsed_ms(z) for a, z in evs]
+            ts += t[1:n_launch - H.refine]
+    finally:
+        L.check(lib.adrt_plan_time_rows_ring(H._plan, None, None, 0), "plan_time_rows_ring")
     return float(np.mean(ts)), ts
 
 
@@ -372,10 +377,10 @@ def main():
     plain32_ms, _, rel_p = batched(H32p)
     H32p.close()
 
-    # dominant kernel: the register row kernel of the G_h pass, events around it inside whole passes
+    # dominant kernel: the register row kernel of the G_h pass, timed by its own dispatch inside whole applies
     ops = H32.ops_op()
     d_tmp = L.DeviceArray((B, N, N))
-    rows_ms, _ = time_rows_kernel(L, H32, d_x32, d_tmp, B, st)
+    rows_ms, rows_ts = time_rows_kernel(L, H32, d_b, d_tmp, B, st)
     fl, nb = rows_kernel_work(H32, B, ops)
     dfl = dense_work(H32, B)[0]
     achieved = fl / (rows_ms * 1e-3)
@@ -384,7 +389,8 @@ def main():
             "frac": achieved / PEAK["fp32"], "traffic": traffic, "traffic_mapping": traffic_map,
             "frac_nonpacked": achieved / PEAK["fp32_nonpacked"],
             "kernel": f"rows_reg_kernel<float, {n}, e, comp> (G_h pass, with the dense MFMA tiles)",
-            "kernel_ms": rows_ms, "flops_per_launch": fl, "alg_bytes_per_launch": nb,
+            "kernel_ms": rows_ms, "kernel_ms_median": float(np.median(rows_ts)), "kernel_ms_min": float(np.min(rows_ts)),
+            "flops_per_launch": fl, "alg_bytes_per_launch": nb,
             "units_per_launch": f"{2 * B} rows x {H32.L - H32.eng.n_dense} non-dense frequencies (+ {2 * B} rows x "
             f"{H32.eng.n_dense} dense frequencies as FP64 MFMA tiles)",
             "dense_fp64_mfma_flops_per_launch": dfl, "dense_fp64_mfma_tflops": dfl / (rows_ms * 1e-3) / 1e12}

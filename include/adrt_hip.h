@@ -183,9 +183,13 @@ int adrt_plan_precond(adrt_plan* plan, const double* d_x, double* d_y, int S, vo
  * plan's precision; 2 / 3 the same in FP64, on rows an FP64 pass left, e.g. after an apply of an FP32 plan with
  * refine >= 1), written to a second buffer: the dominant kernel of an apply, for timing and tuning */
 int adrt_plan_rows_pass(adrt_plan* plan, int which, int S, void* stream);
-/* every later launch of the plan's row kernel is bracketed by records of the two events (NULL, NULL: stop), so the
- * kernel alone can be timed inside a pass; not for a plan whose work is being captured into a graph */
+/* every later launch of the plan's row kernel stamps its own start and end into the two events (hipExtLaunchKernel;
+ * NULL, NULL: stop), so the kernel alone can be timed inside a pass; not for a plan whose work is being captured into
+ * a graph */
 int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end);
+/* the same with count event pairs taken in turn (launch i uses pair i mod count, counting from this call; count 0:
+ * stop), so every row-kernel launch of a whole apply can be timed where it runs */
+int adrt_plan_time_rows_ring(adrt_plan* plan, void* const* ev_begin, void* const* ev_end, int count);
 /* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
 
