@@ -124,3 +124,8 @@ def test_ctx_calls_reject(L):
 
 def test_own_queue_stream_rejects_null(L):
     assert L.load().adrt_stream_create_own_queue(None) == -1 and "null" in err(L)
+
+
+def test_time_rows_ring_rejects(L):
+    lib = L.load()
+    assert lib.adrt_plan_time_rows_ring(None, None, None, 0) == -1 and "null plan" in err(L)
