@@ -170,6 +170,9 @@ __global__ void cert_kernel(int n, int K, const double* omega, int m_lo, double*
 int cert_rowsum_impl(int n, int K, const double* omega, int m_lo, double* out) {
     ADRT_REQUIRE(n >= 1 && n <= 14, "cert_rowsum: need 1 <= n <= 14");
     ADRT_REQUIRE(K >= 0, "cert_rowsum: K must be >= 0");
+    // level m reads r[2a +- 1 + A] for |a| <= 2^(n-m) - 1, inside the 2A + 1 slot only for m >= 1; m_lo > n runs no
+    // level and gives 1, as the reference does
+    ADRT_REQUIRE(m_lo >= 1, "cert_rowsum: need m_lo >= 1");
     if (K == 0) return OK;
     ADRT_REQUIRE(omega && out, "cert_rowsum: null pointer");
     HostCallScratch& s = scratch();
@@ -286,8 +289,19 @@ int adrt_stream_create(void** stream) {
     return OK;
 }
 
+// Live own-queue streams of the process.  Each holds a hardware queue beside HIP's pool (GPU_MAX_HW_QUEUES, 4 by
+// default), so their number is capped: a server needs one per request size it holds.
+std::mutex g_own_mu;
+std::vector<hipStream_t> g_own_queues;
+
 int adrt_stream_create_own_queue(void** stream) {
     ADRT_REQUIRE(stream, "adrt_stream_create_own_queue: null pointer");
+    std::lock_guard<std::mutex> g(g_own_mu);
+    if ((int)g_own_queues.size() >= ADRT_MAX_OWN_QUEUES) {
+        set_error("adrt_stream_create_own_queue: " + std::to_string(ADRT_MAX_OWN_QUEUES) +
+                  " own-queue streams are alive (the cap); destroy one first or use adrt_stream_create");
+        return E_STATE;
+    }
     int dev = 0, cus = 0;
     ADRT_HIP(hipGetDevice(&dev));
     ADRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -297,12 +311,27 @@ int adrt_stream_create_own_queue(void** stream) {
     hipStream_t s;
     // a stream with a CU mask (here: every CU) is given a hardware queue of its own instead of one from the shared pool
     ADRT_HIP(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+    g_own_queues.push_back(s);
     *stream = (void*)s;
     return OK;
 }
 
+int adrt_own_queue_count(void) {
+    std::lock_guard<std::mutex> g(g_own_mu);
+    return (int)g_own_queues.size();
+}
+
 int adrt_stream_destroy(void* stream) {
-    if (stream) ADRT_HIP(hipStreamDestroy((hipStream_t)stream));
+    if (!stream) return OK;
+    {
+        std::lock_guard<std::mutex> g(g_own_mu);
+        for (size_t i = 0; i < g_own_queues.size(); ++i)
+            if (g_own_queues[i] == (hipStream_t)stream) {
+                g_own_queues.erase(g_own_queues.begin() + i);
+                break;
+            }
+    }
+    ADRT_HIP(hipStreamDestroy((hipStream_t)stream));
     return OK;
 }
 

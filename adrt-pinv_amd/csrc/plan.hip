@@ -630,6 +630,20 @@ int adrt_plan_time_rows_ring(adrt_plan* plan, void* const* ev_begin, void* const
     return OK;
 }
 
+int adrt_plan_time_rows_count(adrt_plan* plan, long long* stamped) {
+    auto* p = (Plan*)plan;
+    ADRT_REQUIRE(p && stamped, "plan_time_rows_count: null argument");
+    *stamped = (long long)p->ev_next;
+    return OK;
+}
+
+int adrt_plan_refine_steps(adrt_plan* plan) {
+    auto* p = (Plan*)plan;
+    if (!p) return 0;
+    const int D = p->d.outer_deg;
+    return p->d.precision == PREC_F32 ? (p->d.refine < D ? p->d.refine : D - 1) : 0;
+}
+
 // x = outer Chebyshev of degree D on P G with right-hand side y = backproject(b) (fastops.py:247-258, 287-288, 337-338)
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream) {
     auto* p = (Plan*)plan;
@@ -643,7 +657,7 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     // FP32 plans with refine > 0: the right-hand side and the last `refine` outer steps in FP64 (a mixed-precision
     // refinement: the FP64 steps pull the iterate to the FP64 operator's fixed point)
     const int prec = p->d.precision;
-    const int refine = (prec == PREC_F32) ? (p->d.refine < D ? p->d.refine : D - 1) : 0;
+    const int refine = adrt_plan_refine_steps(plan);
     ADRT_TRY(packed_pass(p, refine > 0 ? PREC_F64 : prec, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st));
     // iterate x_j lands in buffer j % 2, with buffer (D - 1) % 2 == d_x
     double* X[2];

@@ -38,6 +38,7 @@ SIGNATURES = {
     "adrt_host_free": (I, [P]),
     "adrt_stream_create": (I, [ctypes.POINTER(P)]),
     "adrt_stream_create_own_queue": (I, [ctypes.POINTER(P)]),
+    "adrt_own_queue_count": (I, []),
     "adrt_stream_destroy": (I, [P]),
     "adrt_stream_sync": (I, [P]),
     "adrt_stream_wait": (I, [P, P]),
@@ -72,6 +73,8 @@ SIGNATURES = {
     "adrt_plan_rows_pass": (I, [P, I, I, P]),
     "adrt_plan_time_rows": (I, [P, P, P]),
     "adrt_plan_time_rows_ring": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P), I]),
+    "adrt_plan_time_rows_count": (I, [P, ctypes.POINTER(LL)]),
+    "adrt_plan_refine_steps": (I, [P]),
     "adrt_cgls_work_bytes": (ULL, [I, I, I]),
     "adrt_cgls": (I, [I, P, P, I, D, I, I, P, P, P, I, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),
@@ -100,6 +103,7 @@ class PlanDesc(ctypes.Structure):
 
 
 KIND_LP, KIND_HYBRID, KIND_NORMAL = 0, 1, 2
+MAX_OWN_QUEUES = 4  # ADRT_MAX_OWN_QUEUES: own-queue streams alive at once
 PRECOND = {"none": 0, "g1": 1, "lorr": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_plain": 2}
 # the row-kernel configuration table is kept per compute type: both FP32 variants share one entry

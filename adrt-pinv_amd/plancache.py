@@ -6,10 +6,17 @@ Cache directory: $ADRT_PLAN_CACHE, else ~/.cache/adrt-pinv_amd.  Setting ADRT_PL
 """
 
 import hashlib
+import inspect
 import json
 import os
 
 import numpy as np
+
+# Bumped whenever the setup arithmetic (bounds, degrees, outer interval) or this file's format changes, so entries
+# written by an older build are not reused.
+VERSION = 2
+# keywords that do not change the setup scalars of an FP64 plan
+_NOT_SETUP = ("max_batch", "precision", "refine", "ctx", "outer", "lam_max")
 
 
 def cache_dir():
@@ -24,20 +31,43 @@ def cache_dir():
     return d
 
 
+def resolved(cls, **kw):
+    """Every setup keyword of ``cls`` (the given ones over the constructor's defaults), without those that do not
+    change the setup scalars."""
+    sig = inspect.signature(cls.__init__)
+    out = {k: v.default for k, v in sig.parameters.items()
+           if k not in ("self", "n") and v.default is not inspect.Parameter.empty}
+    unknown = set(kw) - set(out)
+    if unknown:
+        raise TypeError(f"{cls.__name__} has no keyword {sorted(unknown)}")
+    out.update(kw)
+    return {k: out[k] for k in sorted(out) if k not in _NOT_SETUP}
+
+
+def _blob(kind, n, params):
+    return json.dumps({"version": VERSION, "kind": kind, "n": n, **params}, sort_keys=True, default=str)
+
+
 def key(kind, n, **params):
-    blob = json.dumps({"kind": kind, "n": n, **{k: params[k] for k in sorted(params)}}, sort_keys=True, default=str)
-    return f"{kind}_n{n}_" + hashlib.sha1(blob.encode()).hexdigest()[:16]
+    return f"{kind}_n{n}_" + hashlib.sha1(_blob(kind, n, params).encode()).hexdigest()[:16]
 
 
 def load(kind, n, **params):
+    """The cached setup of (kind, n, params) or None; an entry whose stored parameters or version differ from the
+    request (a hash collision or a file of another build) is ignored."""
     d = cache_dir()
     if d is None:
         return None
     path = os.path.join(d, key(kind, n, **params) + ".npz")
     if not os.path.exists(path):
         return None
-    z = np.load(path, allow_pickle=False)
-    return {"lam_max": z["lam_max"], "outer": (float(z["lo"]), float(z["hi"]), int(z["D"]))}
+    try:
+        z = np.load(path, allow_pickle=False)
+        if "params" not in z.files or str(z["params"]) != _blob(kind, n, params):
+            return None
+        return {"lam_max": z["lam_max"], "outer": (float(z["lo"]), float(z["hi"]), int(z["D"]))}
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def store(kind, n, lam_max, outer, **params):
@@ -46,15 +76,17 @@ def store(kind, n, lam_max, outer, **params):
         return
     path = os.path.join(d, key(kind, n, **params) + ".npz")
     tmp = path + f".{os.getpid()}.tmp.npz"
-    np.savez(tmp, lam_max=np.asarray(lam_max, dtype=float), lo=outer[0], hi=outer[1], D=outer[2])
+    np.savez(tmp, lam_max=np.asarray(lam_max, dtype=float), lo=outer[0], hi=outer[1], D=outer[2],
+             params=np.array(_blob(kind, n, params)))
     os.replace(tmp, path)
 
 
 def hybrid_setup(n, **kw):
-    """(lam_max, (lo, hi, D)) of FastHybridHip(n, **kw) in FP64, from the cache or computed and stored."""
+    """(lam_max, (lo, hi, D)) of FastHybridHip(n, **kw) in FP64, from the cache or computed and stored.  The key is
+    every setup keyword of FastHybridHip with its default filled in, plus VERSION."""
     from .inverses import FastHybridHip
 
-    params = {k: kw[k] for k in sorted(kw) if k not in ("max_batch", "precision", "refine")}
+    params = resolved(FastHybridHip, **kw)
     hit = load("hybrid", n, **params)
     if hit is not None:
         return hit["lam_max"], hit["outer"]

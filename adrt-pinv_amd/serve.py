@@ -139,6 +139,13 @@ class Server:
                 st = self._streams[(n, j)]
             return GpuSlot(n, cap, precision, refine, mb, setups[(n, cap)], stream=st, **kw)
 
+        if stream_per_size and slot_factory is None:
+            from . import lib as L
+
+            want = len({n for n, _ in self.keys}) * max(1, streams_per_size)
+            if want > L.MAX_OWN_QUEUES:
+                raise ValueError(f"{want} own-queue streams asked for (sizes x streams_per_size); the library allows "
+                                 f"{L.MAX_OWN_QUEUES} at once: serve fewer sizes or set stream_per_size=False")
         factory = slot_factory or gpu_slot
         if mem_budget is None and slot_factory is None:
             from . import lib as L

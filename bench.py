@@ -14,6 +14,7 @@ uneven batch in one graph launch and a Poisson serving trace over mixed N with p
 """
 
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -118,6 +119,7 @@ def time_rows_kernel(L, H, d_b, d_x, S, st, reps=3):
     refinement steps) of each apply."""
     lib = L.load()
     n_launch = H.outer_deg  # one backprojection pass and outer_deg - 1 G_h passes
+    refine = lib.adrt_plan_refine_steps(H._plan)  # FP64 passes at the end: min(refine, outer_deg - 1)
     evs = [(L.Event(), L.Event()) for _ in range(n_launch)]
     b0 = (L.P * n_launch)(*[a.handle for a, _ in evs])
     b1 = (L.P * n_launch)(*[z.handle for _, z in evs])
@@ -129,8 +131,13 @@ def time_rows_kernel(L, H, d_b, d_x, S, st, reps=3):
             L.check(lib.adrt_plan_time_rows_ring(H._plan, b0, b1, n_launch), "plan_time_rows_ring")
             H.apply_device(d_b, d_x, S, st, count=False)
             st.sync()
+            stamped = L.LL()
+            L.check(lib.adrt_plan_time_rows_count(H._plan, ctypes.byref(stamped)), "plan_time_rows_count")
+            # one row-kernel launch per pass, or the slice below would average the wrong launches
+            if stamped.value != n_launch:
+                raise RuntimeError(f"{stamped.value} row-kernel launches stamped in one apply, expected {n_launch}")
             t = [a.elapsed_ms(z) for a, z in evs]
-            ts += t[1:n_launch - H.refine]
+            ts += t[1:n_launch - refine]
     finally:
         L.check(lib.adrt_plan_time_rows_ring(H._plan, None, None, 0), "plan_time_rows_ring")
     return float(np.mean(ts)), ts

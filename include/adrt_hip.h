@@ -71,6 +71,11 @@ int adrt_stream_create(void** stream);
  * behind other streams', e.g. one per request size in a server.  Synchronises with the null stream like
  * hipStreamCreate's streams. */
 int adrt_stream_create_own_queue(void** stream);
+/* At most ADRT_MAX_OWN_QUEUES own-queue streams live at once (each holds a hardware queue beside HIP's pool); one
+ * more fails with -4 until one is destroyed.  adrt_own_queue_count: how many are alive.  (No counterpart in the
+ * reference, which has no device queues.) */
+#define ADRT_MAX_OWN_QUEUES 4
+int adrt_own_queue_count(void);
 int adrt_stream_destroy(void* stream);
 int adrt_stream_sync(void* stream);
 int adrt_stream_wait(void* waiting, void* signal); /* `waiting` waits for the work enqueued so far on `signal` */
@@ -190,6 +195,12 @@ int adrt_plan_time_rows(adrt_plan* plan, void* ev_begin, void* ev_end);
 /* the same with count event pairs taken in turn (launch i uses pair i mod count, counting from this call; count 0:
  * stop), so every row-kernel launch of a whole apply can be timed where it runs */
 int adrt_plan_time_rows_ring(adrt_plan* plan, void* const* ev_begin, void* const* ev_end, int count);
+/* row-kernel launches stamped since the last adrt_plan_time_rows / _ring call (a pass whose rows are all dense
+ * launches no row kernel and stamps nothing), so a caller can check which launches its event pairs hold */
+int adrt_plan_time_rows_count(adrt_plan* plan, long long* stamped);
+/* the outer steps an apply of the plan runs in FP64 at its end: min(refine, outer_deg - 1) for ADRT_PREC_F32, else
+ * 0 (FP32_PLAIN and FP64 plans take no refinement) */
+int adrt_plan_refine_steps(adrt_plan* plan);
 /* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
 

@@ -129,3 +129,32 @@ def test_own_queue_stream_rejects_null(L):
 def test_time_rows_ring_rejects(L):
     lib = L.load()
     assert lib.adrt_plan_time_rows_ring(None, None, None, 0) == -1 and "null plan" in err(L)
+
+
+@pytest.mark.parametrize("m_lo", [0, -1, -40])
+def test_cert_rowsum_rejects_low_levels(L, m_lo):
+    """Level m of the row-sum recursion reads r[2a +- 1 + A] for |a| <= 2^(n-m) - 1: inside its slot only for m >= 1.
+    Below that the call is an argument error before any device work (the reference's loop would read out of bounds)."""
+    om = np.linspace(-3.0, 3.0, 8)
+    out = np.zeros(8)
+    assert L.load().adrt_cert_rowsum(3, 8, L.ptr(om), m_lo, L.ptr(out)) == -1 and "m_lo" in err(L)
+    assert np.all(out == 0)
+
+
+def test_timing_and_refine_queries_reject_null(L):
+    lib = L.load()
+    n = L.LL()
+    assert lib.adrt_plan_time_rows_count(None, ctypes.byref(n)) == -1 and "null" in err(L)
+    assert lib.adrt_plan_refine_steps(None) == 0
+
+
+def test_own_queue_cap_declared(L):
+    """The cap on own-queue streams is part of the header, and no stream is alive before one is made."""
+    import os
+    import re
+
+    from conftest import REPO
+
+    hdr = open(os.path.join(REPO, "include", "adrt_hip.h")).read()
+    assert int(re.search(r"#define ADRT_MAX_OWN_QUEUES (\d+)", hdr).group(1)) == 4
+    assert L.load().adrt_own_queue_count() == 0

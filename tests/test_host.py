@@ -66,6 +66,44 @@ def test_plan_cache_roundtrip(tmp_path, monkeypatch):
     assert C.load("hybrid", 5, dense_cap=16) is None
 
 
+def test_plan_cache_key_resolves_defaults(tmp_path, monkeypatch):
+    """The key holds every setup keyword of the class with its default filled in: a default given explicitly hits
+    the same entry, a changed default (tau_fac, g1_deg, safety ...) misses, and keywords that do not change the setup
+    scalars (precision, max_batch, refine) are left out.  Unknown keywords are an error."""
+    monkeypatch.setenv("ADRT_PLAN_CACHE", str(tmp_path))
+    C = pkg("plancache")
+    I = pkg("inverses")
+    p0 = C.resolved(I.FastHybridHip, dense_cap=8)
+    assert p0 == C.resolved(I.FastHybridHip, dense_cap=8, tau_fac=0.2, g1_deg=14, precision="fp32", max_batch=9)
+    for k in ("tau_fac", "eps_in", "eps", "g1_deg", "safety", "power_iters", "precond", "lam_bound", "dense_cap"):
+        assert k in p0, k
+    for k in ("precision", "max_batch", "refine", "ctx"):
+        assert k not in p0, k
+    C.store("hybrid", 5, np.ones(3), (0.1, 0.9, 17), **p0)
+    assert C.load("hybrid", 5, **p0) is not None
+    assert C.load("hybrid", 5, **C.resolved(I.FastHybridHip, dense_cap=8, tau_fac=0.3)) is None
+    assert C.load("hybrid", 5, **C.resolved(I.FastHybridHip, dense_cap=8, g1_deg=12)) is None
+    with pytest.raises(TypeError):
+        C.resolved(I.FastHybridHip, no_such_keyword=1)
+
+
+def test_plan_cache_ignores_stale_entries(tmp_path, monkeypatch):
+    """An entry under the right file name whose stored parameters differ (written by a build of another VERSION, or
+    by the old format without parameters) is not reused."""
+    monkeypatch.setenv("ADRT_PLAN_CACHE", str(tmp_path))
+    C = pkg("plancache")
+    p = {"dense_cap": 8}
+    path = tmp_path / (C.key("hybrid", 5, **p) + ".npz")
+    np.savez(path, lam_max=np.ones(3), lo=0.1, hi=0.9, D=17)  # old format: no parameters
+    assert C.load("hybrid", 5, **p) is None
+    blob = C._blob("hybrid", 5, p).replace(f'"version": {C.VERSION}', f'"version": {C.VERSION - 1}')
+    assert blob != C._blob("hybrid", 5, p)
+    np.savez(path, lam_max=np.ones(3), lo=0.1, hi=0.9, D=17, params=np.array(blob))
+    assert C.load("hybrid", 5, **p) is None
+    C.store("hybrid", 5, np.ones(3), (0.1, 0.9, 17), **p)
+    assert C.load("hybrid", 5, **p)["outer"] == (0.1, 0.9, 17)
+
+
 class HostSlot:
     """Stand-in for a GPU slot: 'computes' x = sum of the sinogram per request after a fixed number of polls."""
 
