@@ -18,7 +18,7 @@ namespace {
 
 constexpr int LSQ_THREADS = 256;
 constexpr int LSQ_NB = 64;  // reduction blocks per sinogram
-enum { SC_GAMMA = 0, SC_G0 = 1, SC_ALPHA = 2, SC_BETA = 3, SC_ACTIVE = 4, SC_ITERS = 5, SC_N = 8 };
+enum { SC_GAMMA = 0, SC_G0 = 1, SC_ALPHA = 2, SC_BETA = 3, SC_ACTIVE = 4, SC_ITERS = 5, SC_GNEW = 6, SC_N = 8 };
 
 // partial[s][blk] = sum of v[s][i]^2 over the indices i = blk * 256 + t (mod LSQ_NB * 256), tree-reduced in LDS
 __global__ __launch_bounds__(LSQ_THREADS) void dot_partial_kernel(const double* v, long long len, double* partial) {
@@ -59,6 +59,7 @@ __global__ void scalars_kernel(int S, int mode, double tol, const double* partia
         const double it = c[SC_ITERS] + 1.0;
         c[SC_ITERS] = it;
         const double g = sqrt(sum);
+        c[SC_GNEW] = g;
         if (hist_len > 0 && it < hist_len) hist[(size_t)s * hist_len + (long long)it] = g;
         if (g <= tol * c[SC_G0]) {
             c[SC_ACTIVE] = 0.0;
@@ -141,6 +142,11 @@ unsigned long long adrt_cgls_work_bytes(int n, int S, int hist_len) {
 
 int adrt_cgls(int n, const double* d_b, double* d_x, int S, double tol, int maxit, int x0, void* d_work, int* iters,
               double* hist, int hist_len, void* stream) {
+    return adrt_cgls_cb(n, d_b, d_x, S, tol, maxit, x0, d_work, iters, hist, hist_len, stream, nullptr, nullptr);
+}
+
+int adrt_cgls_cb(int n, const double* d_b, double* d_x, int S, double tol, int maxit, int x0, void* d_work,
+                 int* iters, double* hist, int hist_len, void* stream, adrt_cgls_callback callback, void* user) {
     ADRT_REQUIRE(n >= 1 && n <= 13, "cgls: need 1 <= n <= 13");
     ADRT_REQUIRE(S >= 0 && maxit >= 0 && hist_len >= 0, "cgls: negative S, maxit or hist_len");
     ADRT_REQUIRE(tol >= 0.0, "cgls: tol must be >= 0");
@@ -180,7 +186,11 @@ int adrt_cgls(int n, const double* d_b, double* d_x, int S, double tol, int maxi
     };
     bool any = true;
     ADRT_TRY(any_active(any));
+    std::vector<int> ran(callback ? S : 0);
+    std::vector<double> gnorm(callback ? S : 0);
     for (int it = 0; it < maxit && any; ++it) {
+        // the sinograms that run this iteration: those still active after the last one (pinv.py:35-47)
+        for (size_t s = 0; s < ran.size(); ++s) ran[s] = sc[s * SC_N + SC_ACTIVE] != 0.0;
         ADRT_TRY(launch_adrt_forward(n, 2, w.p, w.q, w.aw, S, st));  // q = R p
         ADRT_TRY(dot(w.q, Mb, 1));                                     // alpha = gamma / <q, q>
         hipLaunchKernelGGL(update_xr_kernel, gr, dim3(LSQ_THREADS), 0, st, d_x, w.p, w.r, w.q, w.sc, NN, Mb);
@@ -190,6 +200,11 @@ int adrt_cgls(int n, const double* d_b, double* d_x, int S, double tol, int maxi
         hipLaunchKernelGGL(update_p_kernel, gx, dim3(LSQ_THREADS), 0, st, w.p, w.s, w.sc, NN);
         ADRT_TRY(launch_check("update_p_kernel"));
         ADRT_TRY(any_active(any));
+        if (callback) {
+            // after the stop test, before the next iteration: x and ||R^T r|| of iteration it (pinv.py:43-44)
+            for (int s = 0; s < S; ++s) gnorm[s] = ran[s] ? sc[(size_t)s * SC_N + SC_GNEW] : 0.0;
+            callback(it, d_x, gnorm.data(), ran.data(), S, user);
+        }
     }
     ADRT_TRY(any_active(any));
     for (int s = 0; s < S; ++s) iters[s] = (int)sc[(size_t)s * SC_N + SC_ITERS];

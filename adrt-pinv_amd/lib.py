@@ -77,6 +77,7 @@ SIGNATURES = {
     "adrt_plan_refine_steps": (I, [P]),
     "adrt_cgls_work_bytes": (ULL, [I, I, I]),
     "adrt_cgls": (I, [I, P, P, I, D, I, I, P, P, P, I, P]),
+    "adrt_cgls_cb": (I, [I, P, P, I, D, I, I, P, P, P, I, P, P, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),
     "adrt_graph_begin": (I, [P]),
     "adrt_graph_end": (I, [P, ctypes.POINTER(P)]),
@@ -88,6 +89,10 @@ SIGNATURES = {
     "adrt_layout_permute": (I, [I, P, P, P, I, P]),
     "adrt_transpose": (I, [I, I, P, P, P, I, P]),
 }
+
+
+# adrt_cgls_callback: (it, d_x, gnorm, ran, S, user)
+CGLS_CALLBACK = ctypes.CFUNCTYPE(None, I, P, ctypes.POINTER(D), ctypes.POINTER(I), I, P)
 
 
 class PlanDesc(ctypes.Structure):

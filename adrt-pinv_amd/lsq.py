@@ -21,15 +21,29 @@ import numpy as np
 from . import lib as L
 
 
-def cgls_device(n, d_b, d_x, S, tol=1e-12, maxit=10000, x0=False, hist_len=0, stream=None):
-    """On device arrays: d_b (S, 4, 2N-1, N), d_x (S, N, N).  Returns (iters (S,), hist (S, hist_len) or None)."""
+def cgls_device(n, d_b, d_x, S, tol=1e-12, maxit=10000, x0=False, hist_len=0, stream=None, callback=None):
+    """On device arrays: d_b (S, 4, 2N-1, N), d_x (S, N, N).  Returns (iters (S,), hist (S, hist_len) or None).
+    ``callback(it, d_x_ptr, gnorm (S,), ran (S,))`` is called after every iteration (adrt_cgls_cb)."""
     lib = L.load()
     work = L.DeviceArray((max(1, int(lib.adrt_cgls_work_bytes(n, S, hist_len))),), np.uint8)
     iters = np.zeros(S, dtype=np.int32)
     hist = np.zeros((S, hist_len)) if hist_len else None
-    L.check(lib.adrt_cgls(n, L.ptr(d_b), L.ptr(d_x), S, float(tol), int(maxit), 1 if x0 else 0, work.ptr,
-                          iters.ctypes.data, hist.ctypes.data if hist_len else None, int(hist_len),
-                          stream.handle if stream else None), "cgls")
+    errors = []
+
+    def relay(it, dx, gn, ran, S_, user):
+        if errors:
+            return
+        try:  # an exception cannot cross the C frame: keep it and raise it after the call
+            callback(it, dx, np.ctypeslib.as_array(gn, (S_,)).copy(), np.ctypeslib.as_array(ran, (S_,)).astype(bool))
+        except BaseException as e:
+            errors.append(e)
+
+    cb = L.CGLS_CALLBACK(relay) if callback is not None else None
+    L.check(lib.adrt_cgls_cb(n, L.ptr(d_b), L.ptr(d_x), S, float(tol), int(maxit), 1 if x0 else 0, work.ptr,
+                             iters.ctypes.data, hist.ctypes.data if hist_len else None, int(hist_len),
+                             stream.handle if stream else None, ctypes.cast(cb, L.P) if cb else None, None), "cgls")
+    if errors:
+        raise errors[0]
     return iters, hist
 
 
@@ -49,12 +63,28 @@ def cgls_batch(n, B, tol=1e-12, maxit=10000, X0=None):
 
 
 def cgls(n, b, tol=1e-12, maxit=10000, x0=None, callback=None):
-    """min ||R x - b||_2 (pinv.cgls): returns (x, hist) with hist[j] = ||R^T r_j||.  ``callback`` is not supported
-    (the iteration runs on the device without host round trips of x)."""
-    if callback is not None:
-        raise NotImplementedError("cgls on the GPU does not call back per iteration")
-    X, iters, hists = cgls_batch(n, b, tol, maxit, None if x0 is None else np.asarray(x0)[None])
-    return X[0], hists[0]
+    """min ||R x - b||_2 (pinv.cgls): returns (x, hist) with hist[j] = ||R^T r_j||.  ``callback(it, x, gnorm)`` is
+    called after every iteration, the stopping one included, as pinv.py:43-44 does; ``x`` is a host copy of the
+    iterate (the reference hands over its live array), so a callback that keeps it needs no copy of its own."""
+    if callback is None:
+        X, iters, hists = cgls_batch(n, b, tol, maxit, None if x0 is None else np.asarray(x0)[None])
+        return X[0], hists[0]
+    N = 1 << n
+    B = np.ascontiguousarray(np.asarray(b, dtype=float).reshape(1, 4, 2 * N - 1, N))
+    st = L.Stream()
+    d_b = L.DeviceArray.from_host(B, st)
+    d_x = L.DeviceArray.from_host(np.ascontiguousarray(x0, dtype=float).reshape(1, N, N), st) if x0 is not None \
+        else L.DeviceArray((1, N, N))
+    host = np.empty((N, N))
+
+    def each(it, dx, gnorm, ran):
+        if ran[0]:
+            L.check(L.load().adrt_memcpy(host.ctypes.data, dx, host.nbytes, 2, None), "d2h")
+            callback(it, host.copy(), float(gnorm[0]))
+
+    hl = int(maxit) + 1
+    iters, hist = cgls_device(n, d_b, d_x, 1, tol, maxit, x0 is not None, hl, st, callback=each)
+    return d_x.download(st)[0], hist[0, : iters[0] + 1]
 
 
 # ---------------------------------------------------------------- normal operator R^T R and lstsq_cheb

@@ -235,6 +235,14 @@ int adrt_layout_permute(int n, const int* d_src_of, const double* d_in, double* 
 unsigned long long adrt_cgls_work_bytes(int n, int S, int hist_len);
 int adrt_cgls(int n, const double* d_b, double* d_x, int S, double tol, int maxit, int x0, void* d_work, int* iters,
               double* hist, int hist_len, void* stream);
+/* adrt_cgls with pinv.cgls's per-iteration callback (pinv.py:23, 43-44): after iteration `it` (from 0) has run and
+ * its stream has synchronised, callback(it, d_x, gnorm, ran, S, user) is called on the calling thread.  d_x is the
+ * device iterate (S, N, N) after that iteration, valid until the callback returns; ran[s] = 1 for the sinograms that
+ * ran the iteration (those that had not stopped), gnorm[s] their ||R^T r|| after it (0 where ran[s] = 0).  The
+ * callback may read d_x (e.g. copy it to the host) but must not write it or enqueue work on the stream. */
+typedef void (*adrt_cgls_callback)(int it, const double* d_x, const double* gnorm, const int* ran, int S, void* user);
+int adrt_cgls_cb(int n, const double* d_b, double* d_x, int S, double tol, int maxit, int x0, void* d_work,
+                 int* iters, double* hist, int hist_len, void* stream, adrt_cgls_callback callback, void* user);
 
 #ifdef __cplusplus
 }

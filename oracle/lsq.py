@@ -6,10 +6,11 @@ import numpy as np
 from . import radon
 
 
-def cgls(n, b, tol=1e-12, maxit=10000, x0=None, fwd=None, bwd=None):
+def cgls(n, b, tol=1e-12, maxit=10000, x0=None, fwd=None, bwd=None, callback=None):
     """min ||R x - b||_2 by CGLS; stops when ||R^T r|| <= tol ||R^T b||_0 (pinv.py:23-48).
 
-    ``fwd``/``bwd`` replace the ADRT pair (e.g. with the GPU kernels); they default to the oracle."""
+    ``fwd``/``bwd`` replace the ADRT pair (e.g. with the GPU kernels); they default to the oracle.
+    ``callback(it, x, ||R^T r||)`` after every iteration, before the stop test (pinv.py:43-44)."""
     N = 1 << n
     fwd = fwd or (lambda v: radon.adrt(n, v))
     bwd = bwd or (lambda v: radon.adrt_T(n, v))
@@ -21,7 +22,7 @@ def cgls(n, b, tol=1e-12, maxit=10000, x0=None, fwd=None, bwd=None):
     gamma = float(np.vdot(s, s).real)
     g0 = np.sqrt(gamma)
     hist = [g0]
-    for _ in range(maxit):
+    for it in range(maxit):
         q = fwd(p)
         alpha = gamma / float(np.vdot(q, q).real)
         x += alpha * p
@@ -29,6 +30,8 @@ def cgls(n, b, tol=1e-12, maxit=10000, x0=None, fwd=None, bwd=None):
         s = bwd(r)
         gnew = float(np.vdot(s, s).real)
         hist.append(np.sqrt(gnew))
+        if callback is not None:
+            callback(it, x, np.sqrt(gnew))
         if np.sqrt(gnew) <= tol * g0:
             break
         p = s + (gnew / gamma) * p

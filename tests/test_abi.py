@@ -13,6 +13,7 @@ HEADER = os.path.join(REPO, "include", "adrt_hip.h")
 def header_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"^typedef[^;]*;", "", src, flags=re.M | re.S)  # function-pointer types declare no symbol
     names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", src, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "return")))
 
