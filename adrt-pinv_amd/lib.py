@@ -86,6 +86,8 @@ SIGNATURES = {
     "adrt_work_bytes_fwd": (ULL, [I, I, I]),
     "adrt_work_bytes_T": (ULL, [I, I, I]),
     "adrt_forward": (I, [I, I, P, P, P, I, P]),
+    "adrt_forward_noalloc": (I, [I, I, P, P, I, P]),
+    "adrt_noalloc_scratch_bytes": (ULL, []),
     "adrt_layout_permute": (I, [I, P, P, P, I, P]),
     "adrt_transpose": (I, [I, I, P, P, P, I, P]),
 }

@@ -219,6 +219,19 @@ unsigned long long adrt_work_bytes_T(int n, int dtype, int S);
 int adrt_forward(int n, int dtype, const void* d_x, void* d_b, void* d_work, int S, void* stream);
 int adrt_transpose(int n, int dtype, const void* d_b, void* d_x, void* d_work, int S, void* stream);
 
+/* adrt_forward without a workspace argument, allocating nothing: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference, whose
+ * forward ADRT is NumPy on the host (adrtcore.py:122-128).  The levels use a static device array of
+ * ADRT_NOALLOC_SCRATCH_MIB MiB that every GPU the library runs on holds until the process ends; the sinograms run in
+ * chunks that fit it, so any S works and N up to 2048 in every dtype (a larger N, whose one sinogram needs more,
+ * fails with -1 before any device work; adrt_forward has no such limit).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race, so keep them on one stream or ordered.
+ * No part of the array needs to start at zero (each level writes every entry it later reads).  The results are the
+ * bits adrt_forward gives. */
+#define ADRT_NOALLOC_SCRATCH_MIB 256
+unsigned long long adrt_noalloc_scratch_bytes(void);
+int adrt_forward_noalloc(int n, int dtype, const void* d_x, void* d_b, int S, void* stream);
+
 /* Sinogram layout permutation, the device half of pkg_to_core / core_to_pkg (skimage_adrt_pkg.py:38-51): for each of S
  * float64 sinograms of M = 4 (2N-1) N entries, out[s][i] = in[s][src_of[i]] where src_of[i] >= 0, else 0.  d_src_of:
  * M int32 on the device, each in [-1, M) (the kernel does not check; the host builds the map from fingerprints of

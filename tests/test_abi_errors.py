@@ -158,3 +158,17 @@ def test_own_queue_cap_declared(L):
     hdr = open(os.path.join(REPO, "include", "adrt_hip.h")).read()
     assert int(re.search(r"#define ADRT_MAX_OWN_QUEUES (\d+)", hdr).group(1)) == 4
     assert L.load().adrt_own_queue_count() == 0
+
+
+def test_noalloc_forward_rejects_before_device_work(L):
+    """adrt_forward_noalloc checks its arguments and its per-sinogram scratch need before touching the device: N =
+    4096 needs more than the static scratch in every dtype; S = 0 is a no-op; bad n / dtype / S are errors."""
+    lib = L.load()
+    assert lib.adrt_noalloc_scratch_bytes() == 256 << 20
+    for dtype in (0, 1, 2):
+        assert lib.adrt_forward_noalloc(12, dtype, None, None, 1, None) == -1 and "scratch" in err(L)
+    assert lib.adrt_forward_noalloc(11, 2, None, None, 0, None) == 0
+    assert lib.adrt_forward_noalloc(0, 0, None, None, 1, None) == -1 and "n" in err(L)
+    assert lib.adrt_forward_noalloc(5, 3, None, None, 1, None) == -1 and "dtype" in err(L)
+    assert lib.adrt_forward_noalloc(5, 0, None, None, -1, None) == -1 and "S" in err(L)
+    assert lib.adrt_forward_noalloc(5, 0, None, None, 1, None) == -1 and "null" in err(L)
