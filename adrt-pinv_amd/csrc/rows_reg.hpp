@@ -85,6 +85,41 @@ struct Cfg {
                                                                              : 1;
     __host__ __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
     __device__ static __forceinline__ int pad(int p) { return p + (p >> LOGE); }
+    // phys<K> as a constexpr function, and its inverse: the lane that holds physical index p in grouping K
+    __host__ __device__ static constexpr int phys_c(int K, int lane, int u) {
+        const int B0 = K * LOGE, KB = kb(K);
+        const int q = lane + G * (u >> KB);
+        return ((q >> B0) << (B0 + KB)) | ((u & ((1 << KB) - 1)) << B0) | (q & ((1 << B0) - 1));
+    }
+    __host__ __device__ static constexpr int lane_of(int K, int p) {
+        const int B0 = K * LOGE, KB = kb(K);
+        const int q = (p & ((1 << B0) - 1)) | ((p >> (B0 + KB)) << B0);
+        return q % G;
+    }
+    // Does the transition from grouping KA to KB move values between wavefronts?  Lanes of one row are consecutive
+    // threads, so rows of G <= 64 lanes never do; otherwise a value crosses when its writer and reader lanes lie in
+    // different 64-lane wavefronts.
+    __host__ __device__ static constexpr bool cross(int KA, int KB_) {
+        if (G <= 64) return false;
+        for (int lane = 0; lane < G; ++lane)
+            for (int u = 0; u < E; ++u)
+                if (lane_of(KA, phys_c(KB_, lane, u)) / 64 != lane / 64) return true;
+        return false;
+    }
+    // pass edges K <-> K+1 that cross wavefronts: always the top ones (B0 grows with K)
+    __host__ __device__ static constexpr int first_cross() {
+        for (int K = 0; K + 1 < NPASS; ++K)
+            if (cross(K, K + 1) || cross(K + 1, K)) return K;
+        return NPASS - 1;
+    }
+    static constexpr int FIRSTX = first_cross();
+    __host__ __device__ static constexpr bool cross_on_top() {
+        for (int K = 0; K + 1 < NPASS; ++K)
+            if ((cross(K, K + 1) || cross(K + 1, K)) != (K >= FIRSTX) || cross(K, K + 1) != cross(K + 1, K))
+                return false;
+        return true;
+    }
+    static constexpr int NBUF = FIRSTX < NPASS - 1 ? 2 : 1;  // exchange buffers per row
     // physical index of lane-local element u in the grouping of pass K
     template <int K>
     __device__ static __forceinline__ int phys(int lane, int u) {
@@ -157,15 +192,42 @@ __device__ __forceinline__ void pass_adj(cplx<T> (&v)[1 << LOGE], const cplx<T>*
     }
 }
 
+// Orders a wavefront's LDS accesses before and after it for the compiler; the hardware already performs one
+// wavefront's LDS operations in order, so no wait or barrier instruction is needed between its lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Regroup the row from grouping KA to KB through LDS.  An exchange that stays inside each wavefront needs only
+// wave_sync: every lane writes and reads positions of its own wavefront's lanes, which no other wavefront touches in
+// such an exchange.  An exchange across wavefronts writes, synchronises the block and reads; the edges K <-> K+1
+// that cross (the top ones) alternate between two buffers (up: edge e uses buffer e % 2, down: (e + 1) % 2), so
+// consecutive cross exchanges never share a buffer and the barrier of the next one orders every read of the previous
+// one before the buffer is written again: one barrier per cross exchange instead of two.  X2 is the row's second
+// buffer (null when the configuration has no cross exchange).
 template <typename T, int LOGN, int LOGE, int KA, int KB_>
-__device__ __forceinline__ void transition(cplx<T> (&v)[1 << LOGE], cplx<T>* X, int lane) {
+__device__ __forceinline__ void transition(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx<T>* X2, int lane) {
     using C = Cfg<T, LOGN, LOGE>;
+    constexpr bool X_WAVES = C::cross(KA, KB_);
+    if constexpr (!X_WAVES) {
 #pragma unroll
-    for (int u = 0; u < C::E; ++u) X[C::pad(C::template phys<KA>(lane, u))] = v[u];
-    __syncthreads();
+        for (int u = 0; u < C::E; ++u) X[C::pad(C::template phys<KA>(lane, u))] = v[u];
+        wave_sync();
 #pragma unroll
-    for (int u = 0; u < C::E; ++u) v[u] = X[C::pad(C::template phys<KB_>(lane, u))];
-    __syncthreads();
+        for (int u = 0; u < C::E; ++u) v[u] = X[C::pad(C::template phys<KB_>(lane, u))];
+        wave_sync();
+    } else {
+        constexpr int e = (KA < KB_ ? KA : KB_) - C::FIRSTX;  // cross edge index from the bottom
+        constexpr int buf = KA < KB_ ? (e % 2) : ((e + 1) % 2);
+        cplx<T>* B = buf ? X2 : X;
+#pragma unroll
+        for (int u = 0; u < C::E; ++u) B[C::pad(C::template phys<KA>(lane, u))] = v[u];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < C::E; ++u) v[u] = B[C::pad(C::template phys<KB_>(lane, u))];
+    }
 }
 
 // Slots of the innermost level n (level KB - 1 of the last pass) and their pairs (u0, u0 | HS).
@@ -202,14 +264,15 @@ __device__ __forceinline__ void inner_half(cplx<T> (&v)[1 << LOGE], const cplx<T
 // levels LO..n forward then adjoint (the Gram B^H B), grouping 0 -> grouping 0, HALF the result: the innermost level
 // runs as inner_half.  hh holds h = (z^t + z^(t+1)) / 2 at the innermost level's slots.
 template <typename T, int LOGN, int LOGE, int K, int LO>
-__device__ __forceinline__ void gram_half(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
-                                          const cplx<T> (*tz)[1 << LOGE], const cplx<T>* hh, int lane) {
+__device__ __forceinline__ void gram_half(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx<T>* X2,
+                                          const cplx<T> (*tw)[1 << LOGE], const cplx<T> (*tz)[1 << LOGE],
+                                          const cplx<T>* hh, int lane) {
     using C = Cfg<T, LOGN, LOGE>;
     if constexpr (K + 1 < C::NPASS) {
         pass_fwd<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
-        transition<T, LOGN, LOGE, K, K + 1>(v, X, lane);
-        gram_half<T, LOGN, LOGE, K + 1, LO>(v, X, tw, tz, hh, lane);
-        transition<T, LOGN, LOGE, K + 1, K>(v, X, lane);
+        transition<T, LOGN, LOGE, K, K + 1>(v, X, X2, lane);
+        gram_half<T, LOGN, LOGE, K + 1, LO>(v, X, X2, tw, tz, hh, lane);
+        transition<T, LOGN, LOGE, K + 1, K>(v, X, X2, lane);
         pass_adj<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
     } else {
         pass_fwd<T, LOGN, LOGE, K, LO, LOGN - 1>(v, tw[K], tz[K]);
@@ -220,34 +283,35 @@ __device__ __forceinline__ void gram_half(cplx<T> (&v)[1 << LOGE], cplx<T>* X, c
 
 // adjoint passes K..0 of levels LO..n starting from grouping K
 template <typename T, int LOGN, int LOGE, int K, int LO>
-__device__ __forceinline__ void adj_down(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
-                                         const cplx<T> (*tz)[1 << LOGE], int lane) {
+__device__ __forceinline__ void adj_down(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx<T>* X2,
+                                         const cplx<T> (*tw)[1 << LOGE], const cplx<T> (*tz)[1 << LOGE], int lane) {
     pass_adj<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
     if constexpr (K > 0) {
-        transition<T, LOGN, LOGE, K, K - 1>(v, X, lane);
-        adj_down<T, LOGN, LOGE, K - 1, LO>(v, X, tw, tz, lane);
+        transition<T, LOGN, LOGE, K, K - 1>(v, X, X2, lane);
+        adj_down<T, LOGN, LOGE, K - 1, LO>(v, X, X2, tw, tz, lane);
     }
 }
 
 // forward passes K..NPASS-1 of levels LO..n starting from grouping K, ending in grouping NPASS-1
 template <typename T, int LOGN, int LOGE, int K, int LO>
-__device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, const cplx<T> (*tw)[1 << LOGE],
-                                       const cplx<T> (*tz)[1 << LOGE], int lane) {
+__device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx<T>* X2,
+                                       const cplx<T> (*tw)[1 << LOGE], const cplx<T> (*tz)[1 << LOGE], int lane) {
     pass_fwd<T, LOGN, LOGE, K, LO, LOGN>(v, tw[K], tz[K]);
     if constexpr (K + 1 < Cfg<T, LOGN, LOGE>::NPASS) {
-        transition<T, LOGN, LOGE, K, K + 1>(v, X, lane);
-        fwd_up<T, LOGN, LOGE, K + 1, LO>(v, X, tw, tz, lane);
+        transition<T, LOGN, LOGE, K, K + 1>(v, X, X2, lane);
+        fwd_up<T, LOGN, LOGE, K + 1, LO>(v, X, X2, tw, tz, lane);
     }
 }
 
 template <typename T, int LOGN, int LOGE, bool COMP, int LO>
 __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW)) void rows_reg_kernel(RegArgs RA) {
     using C = Cfg<T, LOGN, LOGE>;
+    static_assert(C::cross_on_top(), "the buffer alternation of transition() needs the cross-wavefront edges on top");
     constexpr int E = C::E;
     constexpr int NP = C::NPASS;
     constexpr int LAST = NP - 1;
     const RowsArgs& A = RA.r;
-    __shared__ cplx<T> Xs[C::RPB * C::ROW];
+    __shared__ cplx<T> Xs[C::NBUF * C::RPB * C::ROW];
     __shared__ int sdeg[C::RPB];
 
     // ---- dense workers: the leading blocks take the pass's dense tiles on the matrix cores while the row blocks
@@ -266,6 +330,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     const int rib = tid / C::G;   // row in block
     const int lane = tid % C::G;  // lane within the row
     cplx<T>* X = Xs + rib * C::ROW;
+    cplx<T>* X2 = C::NBUF > 1 ? X + C::RPB * C::ROW : nullptr;  // second exchange buffer (cross-wavefront edges)
 
     // ---- which row
     long long row;
@@ -361,7 +426,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
         } else if (op == MOP_L1A) {
             pass_adj<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
         } else if (op == MOP_GRAM) {
-            gram_half<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, hh, lane);
+            gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);
 #pragma unroll
             for (int u = 0; u < E; ++u) v[u] = scale((T)2, v[u]);  // exact
         } else if (op == MOP_BTNAT) {
@@ -374,11 +439,11 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
             __syncthreads();
             cplx<T> tzf[NP][E];
             pairs_full(tzf);
-            adj_down<T, LOGN, LOGE, LAST, LO>(v, X, tw, tzf, lane);
+            adj_down<T, LOGN, LOGE, LAST, LO>(v, X, X2, tw, tzf, lane);
         } else if (op == MOP_BEND) {
             cplx<T> tzf[NP][E];
             pairs_full(tzf);
-            fwd_up<T, LOGN, LOGE, 0, LO>(v, X, tw, tzf, lane);
+            fwd_up<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tzf, lane);
 #pragma unroll
             for (int u = 0; u < E; ++u) X[C::pad(C::template phys<LAST>(lane, u))] = v[u];
             __syncthreads();
@@ -425,7 +490,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
                 const c128 wq = wc[it < d ? it : 0];  // loaded before the Gram: its latency hides behind it
 #pragma unroll
                 for (int u = 0; u < E; ++u) v[u] = zi[u];
-                gram_half<T, LOGN, LOGE, 0, LO>(v, X, tw, tz, hh, lane);  // v = M z / 2
+                gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);  // v = M z / 2
                 if (it < d) {
                     // z <- z + (w a) r + (w - 1)(z - z_prev), r = y - M z - tau z, y - M z as one rounding of y - 2 v
                     const T wa = (T)wq.x, w0 = (T)wq.y;
