@@ -68,6 +68,60 @@ __device__ __forceinline__ int lam(int x, int hi) {
     return (x & ~mask) | brev_low(x & mask, hi);
 }
 
+// LDS layout of the pass exchanges.  A row value at physical index p sits at LDS entry addr(p) = sum of w[b] over
+// the set bits b of p: linear in the bits, so the E values a lane exchanges sit at one per-lane base plus
+// compile-time offsets (immediate offsets of ds_read / ds_write).  In pass K a lane holds the E values whose
+// physical indices differ in the pass's element bits [K e, K e + e); the lane's own index bits fill the other
+// positions in the order ord[K]: bit i of the extended lane index q = lane + G * group (the group of the element in a
+// narrower last pass, else 0) goes to physical bit ord[K][i].
+//
+// Default: natural order and w[b] = 2^b + (2^b >> e), i.e. addr(p) = p + (p >> e).  The tuned configurations below
+// (every pass e bits wide) use weights whose 2-adic valuations make every 16-lane group of every grouping reach 16
+// distinct 8-byte bank pairs (addr mod 16), the lane bits 0..3 of each pass being the positions of valuation 0, 1,
+// 2, 3: the 128-byte-per-clock LDS then serves each 64-lane b64 access in four conflict-free cycles, where the
+// default conflicts twice in the last two groupings.  (The default padding has no such solution: three groupings
+// share lane bits pairwise.)  Pass 0 keeps its first six lane bits on physical bits e..e+5, so a wavefront's row
+// load stays one contiguous span, and the passes that exchange across wavefronts are the same ones as in natural
+// order.
+template <int LOGN, int LOGE>
+struct Layout {
+    static constexpr bool custom = false;
+    static constexpr int w[1] = {0};
+    static constexpr int ord[1][1] = {{0}};
+};
+template <>
+struct Layout<8, 2> {  // N = 256, 4 values per lane
+    static constexpr bool custom = true;
+    static constexpr int w[8] = {4, 2, 34, 8, 68, 1, 136, 17};
+    static constexpr int ord[4][6] = {{5, 2, 4, 3, 6, 7}, {5, 1, 0, 6, 4, 7}, {7, 1, 0, 3, 2, 6}, {5, 1, 0, 3, 2, 4}};
+};
+template <>
+struct Layout<10, 2> {  // N = 1024, 4 values per lane
+    static constexpr bool custom = true;
+    static constexpr int w[10] = {8, 1, 2, 4, 136, 68, 34, 17, 289, 578};
+    static constexpr int ord[5][8] = {{7, 2, 3, 4, 5, 6, 8, 9}, {1, 6, 5, 0, 4, 7, 8, 9}, {1, 2, 3, 0, 6, 7, 8, 9},
+                                      {1, 2, 3, 0, 4, 5, 8, 9}, {1, 2, 3, 0, 4, 5, 6, 7}};
+};
+template <>
+struct Layout<9, 3> {  // N = 512, 8 values per lane
+    static constexpr bool custom = true;
+    static constexpr int w[9] = {4, 1, 8, 17, 136, 2, 34, 68, 289};
+    static constexpr int ord[3][6] = {{3, 5, 7, 4, 6, 8}, {1, 6, 0, 2, 7, 8}, {1, 5, 0, 2, 3, 4}};
+};
+template <>
+struct Layout<8, 3> {  // N = 256, 8 values per lane (passes of 3, 3, 2 bits)
+    static constexpr bool custom = true;
+    static constexpr int w[8] = {1, 8, 2, 34, 17, 4, 136, 68};
+    static constexpr int ord[3][6] = {{4, 3, 5, 6, 7, 0}, {0, 2, 7, 1, 6, 0}, {0, 2, 5, 1, 3, 4}};
+};
+template <>
+struct Layout<10, 3> {  // N = 1024, 8 values per lane (passes of 3, 3, 3, 1 bits)
+    static constexpr bool custom = true;
+    static constexpr int w[10] = {2, 8, 1, 4, 136, 17, 34, 578, 68, 289};
+    static constexpr int ord[4][9] = {{5, 6, 3, 4, 7, 8, 9, 0, 0}, {2, 0, 8, 1, 6, 7, 9, 0, 0},
+                                      {2, 0, 3, 1, 4, 5, 9, 0, 0}, {2, 0, 3, 1, 4, 5, 6, 7, 8}};
+};
+
 template <typename T, int LOGN, int LOGE>
 struct Cfg {
     static constexpr int N = 1 << LOGN;
@@ -84,17 +138,62 @@ struct Cfg {
                                 : (sizeof(T) == 4 && LOGE == 3 && LOGN <= 9) ? 3
                                                                              : 1;
     __host__ __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
-    __device__ static __forceinline__ int pad(int p) { return p + (p >> LOGE); }
-    // phys<K> as a constexpr function, and its inverse: the lane that holds physical index p in grouping K
+    using LY = Layout<LOGN, LOGE>;
+    // LDS entry of physical index p (linear in p's bits)
+    __host__ __device__ static constexpr int addr(int p) {
+        if constexpr (LY::custom) {
+            int a = 0;
+            for (int b = 0; b < LOGN; ++b)
+                if ((p >> b) & 1) a += LY::w[b];
+            return a;
+        } else {
+            return p + (p >> LOGE);
+        }
+    }
+    // physical index of lane-local element u in the grouping of pass K (lane + G * (u >> KB) is the lane's index
+    // extended by the element's group in a narrower last pass), and its inverse: the lane holding p in grouping K
     __host__ __device__ static constexpr int phys_c(int K, int lane, int u) {
         const int B0 = K * LOGE, KB = kb(K);
         const int q = lane + G * (u >> KB);
-        return ((q >> B0) << (B0 + KB)) | ((u & ((1 << KB) - 1)) << B0) | (q & ((1 << B0) - 1));
+        if constexpr (LY::custom) {
+            int p = (u & ((1 << KB) - 1)) << B0;
+            for (int i = 0; i < LOGN - KB; ++i)
+                if ((q >> i) & 1) p |= 1 << LY::ord[K][i];
+            return p;
+        } else {
+            return ((q >> B0) << (B0 + KB)) | ((u & ((1 << KB) - 1)) << B0) | (q & ((1 << B0) - 1));
+        }
     }
     __host__ __device__ static constexpr int lane_of(int K, int p) {
         const int B0 = K * LOGE, KB = kb(K);
-        const int q = (p & ((1 << B0) - 1)) | ((p >> (B0 + KB)) << B0);
-        return q % G;
+        if constexpr (LY::custom) {
+            int q = 0;
+            for (int i = 0; i < LOGN - KB; ++i)
+                if ((p >> LY::ord[K][i]) & 1) q |= 1 << i;
+            return q % G;
+        } else {
+            const int q = (p & ((1 << B0) - 1)) | ((p >> (B0 + KB)) << B0);
+            return q % G;
+        }
+    }
+    // the layout is a bijection of the row onto LDS entries below ROW, and phys / lane_of are inverse
+    __host__ __device__ static constexpr bool layout_ok() {
+        bool used[N + (N >> LOGE)] = {};
+        for (int p = 0; p < N; ++p) {
+            const int a = addr(p);
+            if (a < 0 || a >= N + (N >> LOGE) || used[a]) return false;
+            used[a] = true;
+        }
+        for (int K = 0; K < NPASS; ++K) {
+            bool hit[N] = {};
+            for (int lane = 0; lane < G; ++lane)
+                for (int u = 0; u < E; ++u) {
+                    const int p = phys_c(K, lane, u);
+                    if (p < 0 || p >= N || hit[p] || lane_of(K, p) != lane) return false;
+                    hit[p] = true;
+                }
+        }
+        return true;
     }
     // Does the transition from grouping KA to KB move values between wavefronts?  Lanes of one row are consecutive
     // threads, so rows of G <= 64 lanes never do; otherwise a value crosses when its writer and reader lanes lie in
@@ -120,15 +219,15 @@ struct Cfg {
         return true;
     }
     static constexpr int NBUF = FIRSTX < NPASS - 1 ? 2 : 1;  // exchange buffers per row
-    // physical index of lane-local element u in the grouping of pass K
+    // LDS entry of the lane's element 0 in grouping K; element u sits addr(phys_c(K, 0, u)) further on (the lane's
+    // bits and the element's bits are disjoint and addr is linear)
     template <int K>
-    __device__ static __forceinline__ int phys(int lane, int u) {
-        constexpr int B0 = K * LOGE;
-        constexpr int KB = kb(K);
-        const int j = u & ((1 << KB) - 1);
-        const int gs = u >> KB;
-        const int q = lane + G * gs;
-        return ((q >> B0) << (B0 + KB)) | (j << B0) | (q & ((1 << B0) - 1));
+    __device__ static __forceinline__ int base(int lane) {
+        return addr(phys_c(K, lane, 0));
+    }
+    template <int K>
+    __host__ __device__ static constexpr int off(int u) {
+        return addr(phys_c(K, 0, u));
     }
 };
 
@@ -211,23 +310,23 @@ template <typename T, int LOGN, int LOGE, int KA, int KB_>
 __device__ __forceinline__ void transition(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx<T>* X2, int lane) {
     using C = Cfg<T, LOGN, LOGE>;
     constexpr bool X_WAVES = C::cross(KA, KB_);
-    if constexpr (!X_WAVES) {
-#pragma unroll
-        for (int u = 0; u < C::E; ++u) X[C::pad(C::template phys<KA>(lane, u))] = v[u];
-        wave_sync();
-#pragma unroll
-        for (int u = 0; u < C::E; ++u) v[u] = X[C::pad(C::template phys<KB_>(lane, u))];
-        wave_sync();
-    } else {
+    cplx<T>* B = X;
+    if constexpr (X_WAVES) {
         constexpr int e = (KA < KB_ ? KA : KB_) - C::FIRSTX;  // cross edge index from the bottom
         constexpr int buf = KA < KB_ ? (e % 2) : ((e + 1) % 2);
-        cplx<T>* B = buf ? X2 : X;
-#pragma unroll
-        for (int u = 0; u < C::E; ++u) B[C::pad(C::template phys<KA>(lane, u))] = v[u];
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < C::E; ++u) v[u] = B[C::pad(C::template phys<KB_>(lane, u))];
+        if constexpr (buf) B = X2;
     }
+    cplx<T>* W = B + C::template base<KA>(lane);
+    const cplx<T>* R = B + C::template base<KB_>(lane);
+#pragma unroll
+    for (int u = 0; u < C::E; ++u) W[C::template off<KA>(u)] = v[u];
+    if constexpr (X_WAVES)
+        __syncthreads();
+    else
+        wave_sync();
+#pragma unroll
+    for (int u = 0; u < C::E; ++u) v[u] = R[C::template off<KB_>(u)];
+    if constexpr (!X_WAVES) wave_sync();
 }
 
 // Slots of the innermost level n (level KB - 1 of the last pass) and their pairs (u0, u0 | HS).
@@ -307,6 +406,7 @@ template <typename T, int LOGN, int LOGE, bool COMP, int LO>
 __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW)) void rows_reg_kernel(RegArgs RA) {
     using C = Cfg<T, LOGN, LOGE>;
     static_assert(C::cross_on_top(), "the buffer alternation of transition() needs the cross-wavefront edges on top");
+    static_assert(C::layout_ok(), "the exchange layout must be a bijection into the row's LDS buffer");
     constexpr int E = C::E;
     constexpr int NP = C::NPASS;
     constexpr int LAST = NP - 1;
@@ -371,8 +471,8 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
         const int PER = (1 << KB) - 1;
 #pragma unroll
         for (int gs = 0; gs < (E >> KB); ++gs) {
-            const int q = lane + C::G * gs;
-            const int rq = brev_low(q & ((1 << B0) - 1), B0);
+            // the low B0 physical bits of the lane's pairs in this group (below every level of the pass)
+            const int rq = brev_low(C::phys_c(K, lane, gs << KB) & ((1 << B0) - 1), B0);
 #pragma unroll
             for (int i = 0; i < KB; ++i) {
 #pragma unroll
@@ -413,7 +513,9 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
 
     // ---- load the row in natural order (grouping 0: lane holds E consecutive values)
     cplx<T> v[E];
-    const int x0 = lane * E;
+    // natural (= grouping 0 physical) index of the lane's element 0; element u is x0 + u
+    const int x0 = C::phys_c(0, lane, 0);
+    static_assert(C::phys_c(0, 0, E - 1) == E - 1, "grouping 0 holds E consecutive values per lane");
     // (the global row type is T: rows are stored in the compute precision)
     const cplx<T>* Yr = reinterpret_cast<const cplx<T>*>(A.Y) + row * A.in_stride;
 #pragma unroll
@@ -432,10 +534,10 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
         } else if (op == MOP_BTNAT) {
             // natural -> physical order of levels 1..n through LDS, then the adjoint passes down to grouping 0
 #pragma unroll
-            for (int u = 0; u < E; ++u) X[C::pad(lam(x0 + u, LOGN))] = v[u];
+            for (int u = 0; u < E; ++u) X[C::addr(lam(x0 + u, LOGN))] = v[u];
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = X[C::pad(C::template phys<LAST>(lane, u))];
+            for (int u = 0; u < E; ++u) v[u] = X[C::template base<LAST>(lane) + C::template off<LAST>(u)];
             __syncthreads();
             cplx<T> tzf[NP][E];
             pairs_full(tzf);
@@ -445,10 +547,10 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
             pairs_full(tzf);
             fwd_up<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tzf, lane);
 #pragma unroll
-            for (int u = 0; u < E; ++u) X[C::pad(C::template phys<LAST>(lane, u))] = v[u];
+            for (int u = 0; u < E; ++u) X[C::template base<LAST>(lane) + C::template off<LAST>(u)] = v[u];
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = X[C::pad(lam(x0 + u, LOGN))];
+            for (int u = 0; u < E; ++u) v[u] = X[C::addr(lam(x0 + u, LOGN))];
             __syncthreads();
         } else {  // MOP_P
             const int slot = A.dslot ? A.dslot[k] : -1;
@@ -515,7 +617,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
             if (anyd) {
                 // exact Tikhonov filter z = F y in double for dense rows (F^T stored: coalesced over the lane's E)
 #pragma unroll
-                for (int u = 0; u < E; ++u) X[C::pad(x0 + u)] = y[u];
+                for (int u = 0; u < E; ++u) X[C::addr(x0 + u)] = y[u];
                 __syncthreads();
                 if (dense) {
                     const c128* F = A.dFT + (long long)slot * C::N * C::N;
@@ -523,7 +625,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
 #pragma unroll
                     for (int u = 0; u < E; ++u) acc[u] = mk(0.0, 0.0);
                     for (int q = 0; q < C::N; ++q) {
-                        const c128 yq = cast<double>(X[C::pad(q)]);
+                        const c128 yq = cast<double>(X[C::addr(q)]);
                         const c128* Fq = F + (long long)q * C::N + x0;
 #pragma unroll
                         for (int u = 0; u < E; ++u) acc[u] = add(acc[u], mul(Fq[u], yq));
