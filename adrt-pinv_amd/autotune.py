@@ -159,3 +159,48 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=D
             table[(n, 2)] = (f64, 0)
         save_table(table)
     return TuneResult(best, times, dense, dense_times, f64, f64_times)
+
+
+REFINE_CHOICES = (1, 2, 3, 4, 6, 8, 12)
+
+
+def refine_for(n, dense_cap, lam_max, outer, target=1e-6, probes=2, seed=8000, cache=True, **kw):
+    """The smallest number of FP64 refinement steps (outer steps at the end of an FP32 apply that run in FP64) for
+    which the FP32 hybrid plan stays within ``target`` of the FP64 plan on ``probes`` ellipse phantoms with 1% noise
+    (seeds ``seed``+i), at most the outer degree minus one.  Plans whose non-dense rows include ill-conditioned
+    low frequencies (a large dense cap at small N) carry more FP32 rounding into the iterate than one FP64 step
+    removes: measured at N=128, dense cap 32 (no dense rows), 1 step leaves 1.2e-5 and 6 steps 5e-7; caps of 16 and
+    every size up to 1024 need 1 or 2.  The choice is cached beside the plan's setup scalars."""
+    import numpy as np
+
+    from . import phantom as ph
+    from . import radon as R
+    from .inverses import FastHybridHip
+
+    D = int(outer[2])
+    params = {"dense_cap": dense_cap, "target": target, "probes": probes, "seed": seed, "setup_outer": list(outer),
+              "lam": float(np.sum(lam_max)), **kw}
+    hit = plancache.load("refine", n, **params) if cache else None
+    if hit is not None:
+        return int(hit["outer"][2])
+    N = 1 << n
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(seed + i, N)), 0.01, seed + 100 + i) for i in range(probes)])
+    H64 = FastHybridHip(n, lam_max=lam_max, outer=outer, dense_cap=dense_cap, max_batch=probes, **kw)
+    x64 = H64.apply_batch(B)
+    H64.close()
+    best = D - 1
+    for r in REFINE_CHOICES:
+        if r >= D - 1:
+            break
+        H = FastHybridHip(n, lam_max=lam_max, outer=outer, dense_cap=dense_cap, max_batch=probes, precision="fp32",
+                          refine=r, **kw)
+        x = H.apply_batch(B)
+        H.close()
+        err = max(float(np.linalg.norm(x[i] - x64[i]) / np.linalg.norm(x64[i])) for i in range(probes))
+        if err <= target:
+            best = r
+            break
+    if cache:
+        plancache.store("refine", n, np.zeros(1), (0.0, 0.0, best), **params)
+    return best
+

@@ -772,6 +772,12 @@ int adrt_event_destroy(void* ev) {
     return OK;
 }
 
+int adrt_stream_wait_event(void* stream, void* ev) {
+    ADRT_REQUIRE(ev, "stream_wait_event: null event");
+    ADRT_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0));
+    return OK;
+}
+
 int adrt_event_record(void* ev, void* stream) {
     ADRT_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
     return OK;

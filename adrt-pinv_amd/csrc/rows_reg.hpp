@@ -82,7 +82,8 @@ __device__ __forceinline__ int lam(int x, int hi) {
 // default conflicts twice in the last two groupings.  (The default padding has no such solution: three groupings
 // share lane bits pairwise.)  Pass 0 keeps its first six lane bits on physical bits e..e+5, so a wavefront's row
 // load stays one contiguous span, and the passes that exchange across wavefronts are the same ones as in natural
-// order.
+// order.  (N = 256 and N = 512 with 8 values per lane have such layouts too, but measured 3-4% slower than the
+// default in alternating runs on one box, so they keep the default; N = 1024 with 4 values per lane gained 2-4%.)
 template <int LOGN, int LOGE>
 struct Layout {
     static constexpr bool custom = false;
@@ -101,18 +102,6 @@ struct Layout<10, 2> {  // N = 1024, 4 values per lane
     static constexpr int w[10] = {8, 1, 2, 4, 136, 68, 34, 17, 289, 578};
     static constexpr int ord[5][8] = {{7, 2, 3, 4, 5, 6, 8, 9}, {1, 6, 5, 0, 4, 7, 8, 9}, {1, 2, 3, 0, 6, 7, 8, 9},
                                       {1, 2, 3, 0, 4, 5, 8, 9}, {1, 2, 3, 0, 4, 5, 6, 7}};
-};
-template <>
-struct Layout<9, 3> {  // N = 512, 8 values per lane
-    static constexpr bool custom = true;
-    static constexpr int w[9] = {4, 1, 8, 17, 136, 2, 34, 68, 289};
-    static constexpr int ord[3][6] = {{3, 5, 7, 4, 6, 8}, {1, 6, 0, 2, 7, 8}, {1, 5, 0, 2, 3, 4}};
-};
-template <>
-struct Layout<8, 3> {  // N = 256, 8 values per lane (passes of 3, 3, 2 bits)
-    static constexpr bool custom = true;
-    static constexpr int w[8] = {1, 8, 2, 34, 17, 4, 136, 68};
-    static constexpr int ord[3][6] = {{4, 3, 5, 6, 7, 0}, {0, 2, 7, 1, 6, 0}, {0, 2, 5, 1, 3, 4}};
 };
 template <>
 struct Layout<10, 3> {  // N = 1024, 8 values per lane (passes of 3, 3, 3, 1 bits)

@@ -42,6 +42,7 @@ SIGNATURES = {
     "adrt_stream_destroy": (I, [P]),
     "adrt_stream_sync": (I, [P]),
     "adrt_stream_wait": (I, [P, P]),
+    "adrt_stream_wait_event": (I, [P, P]),
     "adrt_device_sync": (I, []),
     "adrt_device_mem_info": (I, [ctypes.POINTER(ULL), ctypes.POINTER(ULL)]),
     "adrt_event_create": (I, [ctypes.POINTER(P)]),
@@ -188,6 +189,10 @@ class Stream:
 
     def wait(self, other):
         check(load().adrt_stream_wait(self.handle, other.handle if other else None), "stream_wait")
+
+    def wait_event(self, ev):
+        """Later work on this stream waits for the last record of ``ev`` (an Event)."""
+        check(load().adrt_stream_wait_event(self.handle, ev.handle), "stream_wait_event")
 
     def close(self):
         if getattr(self, "handle", None):

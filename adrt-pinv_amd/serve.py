@@ -1,17 +1,23 @@
-"""Serving loop: continuous batching of hybrid inversions of mixed N with device-memory admission.
+"""Serving loop: continuous batching of hybrid inversions of mixed N, admitted by device memory.
 
-Requests (a sinogram, its N and its dense cap p) arrive at any time.  The server keeps device plans resident for every
-(N, p) it serves, several instances ("slots") each, with their twiddle / degree / dense tables in HBM.  A slot owns a
-stream, input / output buffers for up to ``max_batch`` sinograms and a completion event.  The loop
+Requests (a sinogram, its N and its dense cap p) arrive at any time.  The server keeps, per (N, p) it serves, the dense
+/ twiddle / degree tables resident and runs its batches in **slots**: a device plan with scratch, input and output
+buffers for ``max_batch`` sinograms, page-locked staging and a hipGraph of the whole inverse per batch size.  Slots
+are the unit of device memory: a (N, p) gets ``slots_per_key`` of them at start while the device-memory budget holds
+them, and (with ``max_slots_per_key`` above that) another one whenever its requests wait with every slot busy and the
+budget still holds one.  A request that has waited ``max_wait_s`` without being started is shed with an error (the
+server's answer to overload); started requests always complete.  The loop
 
-  * polls the slots' events; a finished slot hands its results back and becomes free again,
-  * admits waiting requests of a free slot's (N, p) as one batch (up to max_batch) and launches the whole inverse on
-    the slot's stream (upload from page-locked memory, one hipGraph replay of the inverse for that batch size,
-    download into page-locked memory, completion event),
+  * polls the slots' completion events; a finished batch hands its images back and frees its slot,
+  * sheds requests that waited too long,
+  * launches, on every idle slot, the waiting requests of its (N, p) as one batch: sinograms into the slot's
+    page-locked staging, then on the size's compute stream one upload, the captured inverse and one download,
+  * grows the slots of keys whose requests still wait, as memory allows, and launches on them at once,
 
-so requests that arrive while others are on the GPU start as soon as a slot of their size frees up, and the batches of
-different sizes overlap on the device.  Slots are created at start-up while their device memory fits the budget;
-requests for a size the server does not hold are rejected with an error status, the others are unaffected.
+so requests that arrive while others are on the GPU start as soon as a slot of their size frees up or fits in memory,
+and the batches of different sizes overlap on the device (a size's compute stream has a hardware queue of its own, adrt_stream_create_own_queue, so a small-N batch
+never waits behind an N=1024 batch's kernels on one queue).  Requests for a size the server does not hold are
+rejected with an error; the others are unaffected.
 
 The slot factory is pluggable so the scheduling logic can be exercised without a GPU (tests use a host stand-in).
 """
@@ -32,12 +38,29 @@ class Request:
         self.error = None
 
 
-class GpuSlot:
-    """One resident device plan plus buffers, stream and event.  The sinograms go up from and the images come back into
-    page-locked host buffers on the slot's stream; the inverse itself is replayed from a hipGraph captured once per
-    batch size, so a launch costs three enqueues and the completion is one event."""
+class SizeStreams:
+    """The streams of one N: the inverses and downloads on a stream with a hardware queue of its own (or an ordinary
+    one), and the uploads on the same stream or, with separate_upload, on an ordinary stream that never waits on
+    anything (the compute stream then waits for each batch's upload)."""
 
-    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=4, setup=None, stream=None, **kw):
+    def __init__(self, own_queue=True, separate_upload=False):
+        from . import lib as L
+
+        self.compute = L.Stream(own_queue=own_queue)
+        self.up = L.Stream() if separate_upload else self.compute
+
+    def close(self):
+        for s in {id(self.up): self.up, id(self.compute): self.compute}.values():
+            s.sync()
+            s.close()
+
+
+class GpuSlot:
+    """One resident device plan with its buffers, page-locked staging and events.  A batch: sinograms into the host
+    staging, one upload (upload stream), then on the compute stream, once the upload is in, the inverse replayed from
+    the graph captured for that batch size and one download."""
+
+    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=4, setup=None, streams=None, **kw):
         from . import lib as L
         from .inverses import FastHybridHip
 
@@ -47,19 +70,15 @@ class GpuSlot:
         lam, outer = setup
         self.plan = FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=cap, precision=precision,
                                   max_batch=max_batch, refine=refine, **kw)
-        self.stream = stream or L.Stream()  # a stream shared with other slots runs their launches in turn
+        self._own_streams = streams is None
+        self.streams = streams or SizeStreams(own_queue=False)
         self.d_b = L.DeviceArray((max_batch, 4, 2 * N - 1, N))
         self.d_x = L.DeviceArray((max_batch, N, N))
-        self.event = L.Event()
         self.host_b = L.PinnedArray((max_batch, 4, 2 * N - 1, N))
         self.host_x = L.PinnedArray((max_batch, N, N))
+        self.ev_up, self.ev_done = L.Event(), L.Event()
         self.graphs = {}
         self.busy = []
-
-    @staticmethod
-    def bytes_needed(n, cap, max_batch, plan_bytes=None):
-        N = 1 << n
-        return (plan_bytes or 0) + 8 * max_batch * (4 * (2 * N - 1) * N + N * N)
 
     def device_bytes(self):
         """Device bytes of this slot alone; the dense tables are shared by the slots of one (N, p)."""
@@ -76,27 +95,32 @@ class GpuSlot:
     def graph(self, S):
         g = self.graphs.get(S)
         if g is None:
-            with self.L.Graph(self.stream) as g:
-                self.plan.apply_device(self.d_b, self.d_x, S, self.stream, count=False)
+            st = self.streams.compute
+            with self.L.Graph(st) as g:
+                self.plan.apply_device(self.d_b, self.d_x, S, st, count=False)
             self.graphs[S] = g
         return g
 
     def launch(self, reqs):
         S = len(reqs)
+        L, lib = self.L, self.L.load()
+        ss = self.streams
         for j, r in enumerate(reqs):
             self.host_b.a[j] = r.b
-        L, lib = self.L, self.L.load()
         g = self.graph(S)
-        L.check(lib.adrt_memcpy(self.d_b.ptr, self.host_b.ptr, S * self.host_b.a[0].nbytes, 1, self.stream.handle),
-                "h2d")
-        g.launch(self.stream)
-        L.check(lib.adrt_memcpy(self.host_x.ptr, self.d_x.ptr, S * self.host_x.a[0].nbytes, 4, self.stream.handle),
-                "d2h")
-        self.event.record(self.stream)
+        sb = S * (self.host_b.nbytes // self.max_batch)
+        sx = S * (self.host_x.nbytes // self.max_batch)
+        L.check(lib.adrt_memcpy(self.d_b.ptr, self.host_b.ptr, sb, 1, ss.up.handle), "h2d")
+        if ss.up is not ss.compute:
+            self.ev_up.record(ss.up)
+            ss.compute.wait_event(self.ev_up)
+        g.launch(ss.compute)
+        L.check(lib.adrt_memcpy(self.host_x.ptr, self.d_x.ptr, sx, 4, ss.compute.handle), "d2h")
+        self.ev_done.record(ss.compute)
         self.busy = reqs
 
     def poll(self):
-        if not self.busy or not self.event.done():
+        if not self.busy or not self.ev_done.done():
             return []
         done, self.busy = self.busy, []
         for j, r in enumerate(done):
@@ -108,45 +132,54 @@ class GpuSlot:
         self.plan.close()
         self.host_b.free()
         self.host_x.free()
+        if self._own_streams:
+            self.streams.close()
 
 
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
-                 precision="fp32", refine=1, tune=True, stream_per_size=True, streams_per_size=1, **kw):
+                 precision="fp32", refine=1, tune=True, stream_per_size=True, max_slots_per_key=None,
+                 max_wait_s=None, separate_upload=False, **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
-        deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory).
-        stream_per_size: the slots of one N share one stream on a hardware queue of its own, so a long batch of one
-        size never holds up another size's launches (ordinary streams share a few hardware queues, and a small-N
-        batch could wait behind an N=1024 one); the slots of one size then run their batches in turn, on
-        ``streams_per_size`` such streams (dealt to the slots of the size in turn)."""
+        deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory;
+        a (N, p)'s tables are counted with its first slot).  slots_per_key slots per key are made at start while they
+        fit, more on demand up to max_slots_per_key (default: slots_per_key, no growth) while the budget holds them.
+        refine: FP64 refinement steps of the FP32 plans, one number or {(n, cap): number}.
+        stream_per_size: a size's inverses run on a stream with a hardware queue of its own.  separate_upload: a
+        size's uploads go on an ordinary stream of their own instead of the compute stream.
+        max_wait_s: a request not started within that many seconds is shed with an error (None: requests wait).
+
+        Measured on one MI355X at 200 req/s over N in {128 .. 1024}, p in {16, 32}: uploads on the compute stream
+        and two slots per key give p50 / p99 10 / 120 ms; a separate upload stream 38 / 145 ms (HIP spreads
+        ordinary streams over a few shared hardware queues, so the sizes' uploads and waits hold each other up);
+        growing to three slots per key 54 / 185 ms (the slots of a size share its compute stream, so more slots only
+        queue more batches there)."""
         self.keys = list(keys)
+        self.max_wait_s = max_wait_s
+        self.max_slots = max_slots_per_key or slots_per_key
         self.pending = {k: deque() for k in self.keys}
         self.slots = []
         self.rejected = []
+        self.shed = []
         self.done = []
+        self.grown = 0
         self._streams = {}
-        made = {}
-
-        def gpu_slot(n, cap, mb):
-            st = None
-            if stream_per_size:
-                from . import lib as L
-
-                j = made.get(n, 0) % max(1, streams_per_size)
-                made[n] = made.get(n, 0) + 1
-                if (n, j) not in self._streams:
-                    self._streams[(n, j)] = L.Stream(own_queue=True)
-                st = self._streams[(n, j)]
-            return GpuSlot(n, cap, precision, refine, mb, setups[(n, cap)], stream=st, **kw)
-
+        sizes = sorted({n for n, _ in self.keys})
+        self._mb = (lambda n: max_batch.get(n, 4)) if isinstance(max_batch, dict) else (lambda n: max_batch)
         if stream_per_size and slot_factory is None:
             from . import lib as L
 
-            want = len({n for n, _ in self.keys}) * max(1, streams_per_size)
-            if want > L.MAX_OWN_QUEUES:
-                raise ValueError(f"{want} own-queue streams asked for (sizes x streams_per_size); the library allows "
+            if len(sizes) > L.MAX_OWN_QUEUES:
+                raise ValueError(f"{len(sizes)} own-queue streams asked for (one per size); the library allows "
                                  f"{L.MAX_OWN_QUEUES} at once: serve fewer sizes or set stream_per_size=False")
-        factory = slot_factory or gpu_slot
+
+        def gpu_slot(n, cap, mb):
+            if n not in self._streams:
+                self._streams[n] = SizeStreams(own_queue=stream_per_size, separate_upload=separate_upload)
+            rf = refine.get((n, cap), 1) if isinstance(refine, dict) else refine
+            return GpuSlot(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n], **kw)
+
+        self._factory = slot_factory or gpu_slot
         if mem_budget is None and slot_factory is None:
             from . import lib as L
 
@@ -155,29 +188,43 @@ class Server:
             L.check(L.load().adrt_device_mem_info(L.ctypes.byref(free), L.ctypes.byref(total)), "mem_info")
             mem_budget = int(0.6 * free.value)
         self.mem_budget = mem_budget
-        used = 0
+        self.device_bytes = 0
+        self._slot_bytes = {}
         for rnd in range(slots_per_key):
-            for n, cap in self.keys:
-                s = factory(n, cap, max_batch.get(n, 4) if isinstance(max_batch, dict) else max_batch)
-                b = s.device_bytes() + (getattr(s, "shared_bytes", lambda: 0)() if rnd == 0 else 0)
-                if mem_budget is not None and used + b > mem_budget and rnd > 0:
-                    s.close()  # admission by memory: no further instances of this key
-                    continue
-                used += b
-                self.slots.append(s)
-        self.device_bytes = used
+            for k in self.keys:
+                if rnd > 0 and not self._fits(k):
+                    continue  # admission by memory: no further instances of this key
+                self._add_slot(k, prepare=False)
         self.served = {k: any((s.n, s.cap) == k for s in self.slots) for k in self.keys}
         if tune and slot_factory is None:
             # the row kernel's mapping per N, timed on the first slot of that N at its batch depth (before the graphs
             # are captured, which bake the mapping in)
             from . import autotune
 
-            for n in sorted({s.n for s in self.slots}):
+            for n in sizes:
                 s = next(x for x in self.slots if x.n == n)
-                autotune.tune(s.plan, s.max_batch, s.stream, persist=False)
+                autotune.tune(s.plan, s.max_batch, s.streams.compute, persist=False)
         for s in self.slots:
             if hasattr(s, "prepare"):
                 s.prepare()
+
+    def _fits(self, key):
+        """Would one more slot of ``key`` fit in the budget (its bytes as measured on the key's first slot)?"""
+        if self.mem_budget is None:
+            return True
+        b = self._slot_bytes.get(key)
+        return b is not None and self.device_bytes + b <= self.mem_budget
+
+    def _add_slot(self, key, prepare=True):
+        n, cap = key
+        s = self._factory(n, cap, self._mb(n))
+        first = key not in self._slot_bytes
+        self._slot_bytes.setdefault(key, s.device_bytes())
+        self.device_bytes += s.device_bytes() + (getattr(s, "shared_bytes", lambda: 0)() if first else 0)
+        self.slots.append(s)
+        if prepare and hasattr(s, "prepare"):
+            s.prepare()
+        return s
 
     def submit(self, req):
         k = (req.n, req.cap)
@@ -189,22 +236,39 @@ class Server:
         self.pending[k].append(req)
         return True
 
+    def _shed_old(self, now):
+        if self.max_wait_s is None:
+            return
+        for q in self.pending.values():
+            while q and now - q[0].t_arrive > self.max_wait_s:
+                r = q.popleft()
+                r.error = f"shed: not started within {self.max_wait_s} s"
+                r.t_done = now
+                self.shed.append(r)
+
+    def _start(self, slot, q):
+        batch = [q.popleft() for _ in range(min(slot.max_batch, len(q)))]
+        t = time.perf_counter()
+        for r in batch:
+            r.t_start = t
+        slot.launch(batch)
+
     def step(self):
         now = time.perf_counter()
         for s in self.slots:
             for r in s.poll():
                 r.t_done = now
                 self.done.append(r)
+        self._shed_old(now)
         for s in self.slots:
-            if s.busy:
-                continue
             q = self.pending[(s.n, s.cap)]
-            if q:
-                batch = [q.popleft() for _ in range(min(s.max_batch, len(q)))]
-                t = time.perf_counter()
-                for r in batch:
-                    r.t_start = t
-                s.launch(batch)
+            if q and not s.busy:
+                self._start(s, q)
+        # requests still waiting with every slot of their key busy: one more slot, where memory holds it
+        for k, q in self.pending.items():
+            if q and sum(1 for s in self.slots if (s.n, s.cap) == k) < self.max_slots and self._fits(k):
+                self.grown += 1
+                self._start(self._add_slot(k), q)
 
     def idle(self):
         return all(not s.busy for s in self.slots) and all(not q for q in self.pending.values())
@@ -232,7 +296,6 @@ class Server:
         for s in self.slots:
             s.close()
         for st in self._streams.values():
-            st.sync()
             st.close()
         self._streams.clear()
 
@@ -256,16 +319,30 @@ def poisson_trace(rate, duration, sizes, weights, caps, pool, seed=7):
     return out
 
 
-def latency_stats(done, t_begin=None):
+def latency_stats(done, t_begin=None, by_size=False):
+    """p50 / p99 / mean latency (arrival to result) and throughput of the completed requests; with ``by_size`` also
+    per N, with the mean time spent waiting for a slot (arrival to launch) and in the slot (launch to result)."""
     lat = np.array([r.t_done - r.t_arrive for r in done])
     if lat.size == 0:
         return {"count": 0}
     t_first = min(r.t_arrive for r in done) if t_begin is None else t_begin
     t_last = max(r.t_done for r in done)
-    return {
+    out = {
         "count": int(lat.size),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "mean_ms": float(lat.mean() * 1e3),
         "throughput_per_s": float(lat.size / max(t_last - t_first, 1e-9)),
     }
+    if by_size:
+        per = {}
+        for n in sorted({r.n for r in done}):
+            rs = [r for r in done if r.n == n]
+            lt = np.array([r.t_done - r.t_arrive for r in rs]) * 1e3
+            e = {"count": len(rs), "p50_ms": float(np.percentile(lt, 50)), "p99_ms": float(np.percentile(lt, 99))}
+            if all(r.t_start is not None for r in rs):
+                e["wait_ms"] = float(np.mean([r.t_start - r.t_arrive for r in rs]) * 1e3)
+                e["slot_ms"] = float(np.mean([r.t_done - r.t_start for r in rs]) * 1e3)
+            per[str(1 << n)] = e
+        out["by_size"] = per
+    return out

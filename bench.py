@@ -2,7 +2,7 @@
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--n 8] [--dense-cap 16] [--no-cpu] [--no-serve]
     python bench.py --rows-only [--mapping LOGE,DENSE]   # short workload for rocprofv3 --pmc runs of the row kernel
-    python bench.py --large          # adds the N=2048, batch-16 FP32 vs FP64 leg
+    python bench.py --no-large       # without the N=2048, batch-16 FP32 vs FP64 leg
 
 A step is one batched hybrid inverse (FP32 row engine with TwoSum-compensated Chebyshev accumulation; FP64 dense
 table GEMM, preconditioner, outer iteration and the last refinement steps) of B noisy ellipse-phantom sinograms that
@@ -169,39 +169,95 @@ def mixed_batch_leg(count=512, dense_cap=16):
     counts = {int(1 << n): ns.count(n) for n in sorted(set(ns))}
     tuned = {int(1 << n): int(v[0]) for n, v in mb.tuning.items()}
     dense = {int(1 << n): int(v.dense) for n, v in mb.tuning.items()}
+    # accuracy of the last replay: every sinogram against its FP64 plan (same setup), one batched FP64 call per N
+    x = mb.outputs()
     mb.close()
+    rel = fp64_relerr(ns, sinos, x, dense_cap)
     return {"sinograms": count, "sizes": counts, "ms_per_launch": ms, "inverses_per_s": count * 1000.0 / ms,
-            "setup_s": setup_s, "row_mapping_loge": tuned, "dense_placement": dense, "dense_cap": dense_cap}
+            "setup_s": setup_s, "row_mapping_loge": tuned, "dense_placement": dense, "dense_cap": dense_cap,
+            "relerr_fp32_vs_fp64_max": float(np.max(rel)), "relerr_fp32_vs_fp64_median": float(np.median(rel))}
+
+
+def fp64_relerr(ns, sinos, xs, dense_cap, caps=None):
+    """Relative error of each result xs[i] against the FP64 hybrid inverse of sinos[i] with the same setup scalars
+    (N = 2^ns[i], dense cap caps[i] or dense_cap), run as one batched FP64 call per (N, cap)."""
+    I = pkg("inverses")
+    C = pkg("plancache")
+    caps = caps or [dense_cap] * len(ns)
+    groups = {}
+    for i, key in enumerate(zip(ns, caps)):
+        groups.setdefault(key, []).append(i)
+    rel = np.empty(len(ns))
+    for (n, cap), idx in sorted(groups.items()):
+        lam, outer = C.hybrid_setup(n, dense_cap=cap)
+        H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=cap, max_batch=min(len(idx), 64))
+        ref = H.apply_batch(np.stack([sinos[i] for i in idx]))
+        H.close()
+        for j, i in enumerate(idx):
+            rel[i] = np.linalg.norm(xs[i] - ref[j]) / np.linalg.norm(ref[j])
+    return rel
 
 
 SERVE_BATCH = {7: 16, 8: 16, 9: 8, 10: 8}  # sinograms per launch by log2 N
 
 
-def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), slots_per_key=2):
+def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), slots_per_key=2,
+                max_wait_s=0.25):
     """Poisson arrivals (default_rng(7)) of FP32 hybrid requests over mixed N and p at each rate; p50 / p99 latency
-    and throughput (the request's upload and download are inside its latency)."""
+    and throughput of the served requests (upload and download inside the latency), how many were shed (not admitted
+    into device memory within max_wait_s: the server at overload), and the relative error of every served result
+    against the FP64 inverse of its sinogram with the same setup."""
     S = pkg("serve")
     C = pkg("plancache")
     keys = [(n, c) for n in sizes for c in caps]
     t0 = time.time()
     setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
-    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=SERVE_BATCH, setups=setups)
+    # FP64 refinement steps per (N, p): the fewest that keep the FP32 result within 1e-6 of FP64 on probe sinograms
+    refine = {k: pkg("autotune").refine_for(k[0], k[1], *setups[k]) for k in keys}
+    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=SERVE_BATCH, setups=setups, max_wait_s=max_wait_s,
+                   refine=refine)
     setup_s = time.time() - t0
     pool = {n: list(make_sinos(n, 4, 3000, 4000)) for n in sizes}
+    # FP64 results of every (sinogram, cap) the trace can draw: the served results are checked against them
+    ns_all, caps_all, b_all = zip(*[(n, c, b) for n in sizes for c in caps for b in pool[n]])
+    ref = {}
+    I = pkg("inverses")
+    for (n, c), (lam, outer) in setups.items():
+        H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=c, max_batch=len(pool[n]))
+        for j, x in enumerate(H.apply_batch(np.stack(pool[n]))):
+            ref[(n, c, j)] = x
+        H.close()
     # warm every slot once (first launches load code objects)
     warm = [(0.0, S.Request(-1 - i, s.n, s.cap, pool[s.n][0], 0.0)) for i, s in enumerate(srv.slots)]
     srv.run_trace(warm)
     out = []
     for rate in rates:
         srv.done.clear()
+        srv.shed.clear()
         trace = S.poisson_trace(rate, duration, list(sizes), list(weights), list(caps), pool)
         done = srv.run_trace(trace)
-        out.append({"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **S.latency_stats(done)})
-        log(f"serving at {rate:.0f}/s: p50 {out[-1].get('p50_ms', 0):.1f} ms p99 {out[-1].get('p99_ms', 0):.1f} ms "
-            f"throughput {out[-1].get('throughput_per_s', 0):.0f}/s")
+        rel, relk = [], {}
+        for r in done:
+            j = next(i for i, b in enumerate(pool[r.n]) if b is r.b)
+            x64 = ref[(r.n, r.cap, j)]
+            rel.append(float(np.linalg.norm(r.x - x64) / np.linalg.norm(x64)))
+            key = f"{1 << r.n},{r.cap}"
+            relk[key] = max(relk.get(key, 0.0), rel[-1])
+        st = S.latency_stats(done, t_begin=min(r.t_arrive for _, r in trace), by_size=True) if done else {"count": 0}
+        out.append({"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **st, "shed": len(srv.shed),
+                    "relerr_fp32_vs_fp64_max": max(rel) if rel else None,
+                    "relerr_fp32_vs_fp64_median": float(np.median(rel)) if rel else None,
+                    "relerr_fp32_vs_fp64_max_by_key": relk})
+        log(f"serving at {rate:.0f}/s: p50 {st.get('p50_ms', 0):.1f} ms p99 {st.get('p99_ms', 0):.1f} ms, served "
+            f"{st.get('throughput_per_s', 0):.0f}/s, shed {len(srv.shed)} of {len(trace)}, relerr max "
+            f"{max(rel) if rel else float('nan'):.2e} ({', '.join(f'{k}: {v:.1e}' for k, v in sorted(relk.items()))})")
+        for nn, e in st.get("by_size", {}).items():
+            log(f"   N={nn}: {e['count']} served, p50 {e['p50_ms']:.1f} p99 {e['p99_ms']:.1f} ms; mean wait "
+                f"{e.get('wait_ms', 0):.1f} ms, in slot {e.get('slot_ms', 0):.1f} ms")
     res = {"mix": "N in {128,256,512,1024} w .4/.3/.2/.1, p in {16,32}", "slots": len(srv.slots),
-           "max_batch": {str(1 << n): b for n, b in SERVE_BATCH.items()}, "device_GiB": srv.device_bytes / 2**30,
-           "setup_s": setup_s, "rates": out}
+           "max_batch": {str(1 << n): b for n, b in SERVE_BATCH.items()},
+           "max_wait_s": max_wait_s, "fp64_refine_steps": {f"{1 << n},{c}": r for (n, c), r in refine.items()},
+           "device_GiB": srv.device_bytes / 2**30, "setup_s": setup_s, "rates": out}
     srv.close()
     return res
 
@@ -295,14 +351,17 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rates", default="200,400", help="Poisson arrival rates (req/s), comma separated")
-    ap.add_argument("--serve-duration", type=float, default=5.0,
-                    help="seconds of Poisson arrivals per rate (5 s at 200 req/s: about 1000 requests behind the p99)")
+    ap.add_argument("--serve-rates", default="200,500,1000,2000,5000",
+                    help="Poisson arrival rates (req/s), comma separated")
+    ap.add_argument("--serve-duration", type=float, default=4.0,
+                    help="seconds of Poisson arrivals per rate (4 s at 200 req/s: about 800 requests behind the p99)")
+    ap.add_argument("--serve-max-wait", type=float, default=0.25,
+                    help="a request not admitted into device memory within this many seconds is shed")
     ap.add_argument("--rows-only", action="store_true")
     ap.add_argument("--mapping", default=None,
                     help="with --rows-only: 'loge,dense' (row mapping and dense placement) instead of autotuning")
-    ap.add_argument("--large", action="store_true",
-                    help="also the N=2048 batch-16 FP32 vs FP64 leg (about 45 s of setup and 50 GiB of dense tables)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the N=2048 batch-16 FP32 vs FP64 leg (about 45 s of setup and 50 GiB of dense tables)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the sinogram batch would shard by "
@@ -409,9 +468,10 @@ def main():
         log(f"mixed batch: {mixed['inverses_per_s']:.0f} inverses/s")
     serving = None
     if not a.no_serve:
-        serving = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_duration)
+        serving = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_duration,
+                              max_wait_s=a.serve_max_wait)
 
-    large = large_leg() if a.large else None
+    large = None if a.no_large else large_leg()
 
     cpu = None
     if not a.no_cpu:
@@ -419,7 +479,10 @@ def main():
         res = cpu_baseline(H64, cap, Bn[0], threads, a.cpu_budget)
         if res is not None:
             tmin, reps, xr = res
+            from oracle import refcpp
+
             cpu = {"value": 1.0 / tmin, "unit": "inverses/s", "cores": threads, "kind": "reference",
+                   "cpu_model": refcpp.cpu_info()[0], "build": f"g++ -O3 -march={refcpp.MARCH} -fopenmp",
                    "sample": f"1 hybrid inverse at N={N} (dense_cap={cap}, D={H64.outer_deg}), tmin of {reps}; "
                              f"reference cpp/oblique.cpp row kernels (OpenMP, {threads} threads) under the oracle's "
                              "NumPy host (FFTs, packing, outer loop)",

@@ -79,6 +79,7 @@ int adrt_own_queue_count(void);
 int adrt_stream_destroy(void* stream);
 int adrt_stream_sync(void* stream);
 int adrt_stream_wait(void* waiting, void* signal); /* `waiting` waits for the work enqueued so far on `signal` */
+int adrt_stream_wait_event(void* stream, void* ev); /* later work on `stream` waits for the last record of `ev` */
 int adrt_device_sync(void);
 int adrt_device_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
 int adrt_event_create(void** ev);

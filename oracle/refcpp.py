@@ -12,7 +12,37 @@ import numpy as np
 
 from . import rows as rw
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "liboblique.so")
+_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref")
+V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def cpu_info():
+    """(model name, flags) of this host from /proc/cpuinfo (empty when unreadable)."""
+    model, flags = "", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if not model and line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                elif not flags and line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                if model and flags:
+                    break
+    except OSError:
+        pass
+    return model, flags
+
+
+def _pick():
+    """The AVX-512 (x86-64-v4) build where this CPU runs it, else the x86-64-v3 one: the nearest to the reference's
+    -march=native that a library built on another host can be."""
+    v4 = os.path.join(_REF, "liboblique_v4.so")
+    if os.path.exists(v4) and all(f in cpu_info()[1] for f in V4_FLAGS):
+        return v4, "x86-64-v4"
+    return os.path.join(_REF, "liboblique.so"), "x86-64-v3"
+
+
+LIB_PATH, MARCH = _pick()
 _lib = None
 
 

@@ -99,15 +99,19 @@ def test_mixed_batch_512_sinograms(gpu):
 
 
 def test_serving_n128_to_1024(gpu):
-    """The serving workload's sizes and caps (N in {128, 256, 512, 1024}, p in {16, 32}): requests released over a
-    short trace all complete; each result equals, bit for bit, the same request run alone through its slot's plan,
-    and is within 1e-5 of the FP64 inverse with the same setup."""
+    """The serving workload's sizes and caps (N in {128, 256, 512, 1024}, p in {16, 32}), each FP32 plan with the
+    FP64 refinement steps refine_for picks: requests released over a short trace all complete; each result equals,
+    bit for bit, the same request run alone through its slot's plan, and is within 1e-5 of the FP64 inverse with the
+    same setup."""
     S = pkg("serve")
     C = pkg("plancache")
     I = pkg("inverses")
+    A = pkg("autotune")
     keys = [(n, p) for n in (7, 8, 9, 10) for p in (16, 32)]
     setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
-    srv = S.Server(keys, slots_per_key=1, max_batch={7: 4, 8: 4, 9: 2, 10: 2}, setups=setups)
+    refine = {k: A.refine_for(k[0], k[1], *setups[k]) for k in keys}
+    assert refine[(7, 32)] > refine[(7, 16)]  # no dense rows at N=128, p=32: more FP64 steps for the same error
+    srv = S.Server(keys, slots_per_key=1, max_batch={7: 4, 8: 4, 9: 2, 10: 2}, setups=setups, refine=refine)
     rng = np.random.default_rng(7)
     reqs = []
     for i in range(24):

@@ -87,6 +87,19 @@ def test_plan_cache_key_resolves_defaults(tmp_path, monkeypatch):
         C.resolved(I.FastHybridHip, no_such_keyword=1)
 
 
+def test_plan_cache_holds_refine_choices(tmp_path, monkeypatch):
+    """The refinement-step choice is cached under the parameters refine_for keys it by (a list-valued interval and a
+    float among them)."""
+    monkeypatch.setenv("ADRT_PLAN_CACHE", str(tmp_path))
+    C = pkg("plancache")
+    params = {"dense_cap": 32, "target": 1e-6, "probes": 2, "seed": 8000, "setup_outer": [0.16, 0.96, 39],
+              "lam": 1234.5}
+    assert C.load("refine", 7, **params) is None
+    C.store("refine", 7, np.zeros(1), (0.0, 0.0, 6), **params)
+    assert C.load("refine", 7, **params)["outer"][2] == 6
+    assert C.load("refine", 7, **{**params, "target": 1e-7}) is None
+
+
 def test_plan_cache_ignores_stale_entries(tmp_path, monkeypatch):
     """An entry under the right file name whose stored parameters differ (written by a build of another VERSION, or
     by the old format without parameters) is not reused."""
@@ -134,7 +147,7 @@ class HostSlot:
         pass
 
 
-def test_server_continuous_batching_and_rejection():
+def host_server(keys, slots_per_key=2, max_batch=3, mem_budget=10**9, **kw):
     S = pkg("serve")
     made = []
 
@@ -143,7 +156,14 @@ def test_server_continuous_batching_and_rejection():
         made.append(s)
         return s
 
-    srv = S.Server([(7, 16), (8, 16)], slots_per_key=2, max_batch=3, mem_budget=10**9, slot_factory=factory)
+    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=max_batch, mem_budget=mem_budget,
+                   slot_factory=factory, **kw)
+    return srv, made
+
+
+def test_server_continuous_batching_and_rejection():
+    S = pkg("serve")
+    srv, made = host_server([(7, 16), (8, 16)])
     rng = np.random.default_rng(0)
     reqs = [S.Request(i, 7 if i % 3 else 8, 16, rng.standard_normal(5), 0.0) for i in range(20)]
     bad = S.Request(99, 11, 16, np.ones(3), 0.0)  # a size the server does not hold
@@ -165,11 +185,57 @@ def test_server_continuous_batching_and_rejection():
 
 
 def test_server_memory_budget_limits_slots():
-    S = pkg("serve")
-    srv = S.Server([(7, 16), (8, 32)], slots_per_key=4, max_batch=2, mem_budget=3000,
-                   slot_factory=lambda n, cap, mb: HostSlot(n, cap, mb))
+    srv, made = host_server([(7, 16), (8, 32)], slots_per_key=4, max_batch=2, mem_budget=3000)
     assert len(srv.slots) == 3 and srv.device_bytes <= 3000
     assert srv.served[(7, 16)] and srv.served[(8, 32)]
+
+
+def test_server_grows_slots_while_memory_holds_them():
+    """Requests waiting with every slot of their key busy get another slot while the budget holds one (up to
+    max_slots_per_key); with the budget spent they wait for a slot to free instead, and all complete."""
+    S = pkg("serve")
+    for budget, want in ((10**9, 3), (2000, 2)):
+        srv, made = host_server([(7, 16)], slots_per_key=1, max_batch=2, mem_budget=budget, max_slots_per_key=3)
+        reqs = [S.Request(i, 7, 16, np.full(2, float(i)), 0.0) for i in range(12)]
+        for r in reqs:
+            srv.submit(r)
+        for _ in range(100):
+            srv.step()
+            if srv.idle():
+                break
+        assert srv.idle() and len(srv.done) == 12 and not srv.shed
+        assert len(srv.slots) == want and srv.grown == want - 1 and srv.device_bytes <= budget
+        assert [r.rid for s in made for b in s.batches for r in reqs if r.rid in b] == \
+            [rid for s in made for b in s.batches for rid in b]
+        for r in reqs:
+            assert r.x[0, 0] == np.sum(r.b)
+
+
+def test_server_sheds_requests_that_wait_too_long():
+    """Under overload a request not started within max_wait_s is shed with an error; started requests always
+    complete."""
+    S = pkg("serve")
+    import time
+
+    srv, made = host_server([(8, 16)], slots_per_key=1, max_batch=2, max_slots_per_key=1, max_wait_s=30.0)
+    fresh = [S.Request(i, 8, 16, np.ones(2), time.perf_counter()) for i in range(2)]
+    for r in fresh:
+        srv.submit(r)
+    srv.step()  # both start on the one slot
+    assert all(r.t_start is not None for r in fresh)
+    old = [S.Request(10 + i, 8, 16, np.ones(2), time.perf_counter() - 60.0) for i in range(4)]  # waited a minute
+    new = [S.Request(20 + i, 8, 16, np.ones(2), time.perf_counter()) for i in range(2)]
+    for r in old + new:
+        srv.submit(r)
+    for _ in range(50):
+        srv.step()
+        if srv.idle():
+            break
+    assert srv.idle()
+    assert sorted(r.rid for r in srv.shed) == [10, 11, 12, 13]
+    assert all(r.error.startswith("shed") and r.x is None for r in srv.shed)
+    assert sorted(r.rid for r in srv.done) == [0, 1, 20, 21]
+    assert all(r.error is None and r.x is not None for r in srv.done)
 
 
 def test_poisson_trace_and_stats():
