@@ -257,6 +257,7 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     ADRT_REQUIRE(d->max_batch >= 1, "plan_create: max_batch must be >= 1");
     ADRT_REQUIRE(d->refine >= 0, "plan_create: refine must be >= 0");
     ADRT_REQUIRE(d->dense_on_device >= 0 && d->dense_on_device <= 2, "plan_create: dense_on_device must be 0, 1 or 2");
+    ADRT_REQUIRE(d->g1_fp64 == 0 || d->g1_fp64 == 1, "plan_create: g1_fp64 must be 0 or 1");
     ADRT_REQUIRE(d->z && ((d->deg && d->a && d->c) || d->kind == ADRT_KIND_NORMAL),
                  "plan_create: null row parameter array");
     ADRT_REQUIRE(d->kind != ADRT_KIND_NORMAL || d->n_dense == 0, "plan_create: the normal operator has no dense rows");
@@ -669,7 +670,7 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     const double* py = p->y;
     if (fused) {
         ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, p->y, nullptr, nullptr, a, 0.0, 0.0, X[0], S, st,
-                                 refine > 0 ? PREC_F64 : prec));
+                                 (refine > 0 || p->d.g1_fp64) ? PREC_F64 : prec));
     } else {
         if (has_p) {
             ADRT_TRY(precond(p, p->y, p->r, S, st));
@@ -683,12 +684,14 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
         const double* xk = X[(k - 1) % 2];
         const double* xp = (k == 1) ? nullptr : X[k % 2];
         const int pk = (refine > 0 && k >= D - refine) ? PREC_F64 : prec;
+        // the G_1 iteration of an FP32 step runs in float unless the plan asks for double (g1_fp64)
+        const int pg = p->d.g1_fp64 ? PREC_F64 : pk;
         // an FP32 step's residual feeds only the float G_1 iteration, which rounds it to float: store it as float
-        const bool res32 = fused && pk == PREC_F32;
+        const bool res32 = fused && pg == PREC_F32;
         ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st, res32));
         if (fused) {
             ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, res32 ? nullptr : p->res, xk, xp, a, om, om - 1.0,
-                                     X[k % 2], S, st, pk, res32 ? reinterpret_cast<const float*>(p->res) : nullptr));
+                                     X[k % 2], S, st, pg, res32 ? reinterpret_cast<const float*>(p->res) : nullptr));
             continue;
         }
         const double* rr = p->res;

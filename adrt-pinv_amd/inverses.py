@@ -13,6 +13,8 @@ eigensolve) touches the host.  Extra keywords:
   max_batch  sinograms per device call (``apply_batch``)
   outer      (lo, hi, D) to skip the eigensolve (e.g. to reuse the reference's setup scalars)
   refine     with precision 'fp32': the backprojection and the last ``refine`` outer steps run in FP64
+  g1_fp64    (hybrid) with precision 'fp32': the G_1 preconditioner's inner iteration runs in double in the FP32
+             outer steps too (default: in float there)
   ctx        a lib.Context whose device-memory budget the plan is charged to
 """
 
@@ -79,6 +81,7 @@ class _PlanBase:
         d.dense_Wback = L.ptr(Wback)
         d.refine = int(self.refine)
         d.dense_on_device = 2
+        d.g1_fp64 = int(bool(getattr(self, "g1_fp64", False)))
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         ctx = getattr(self, "ctx", None)
@@ -236,8 +239,9 @@ class FastHybridHip(_PlanBase):
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None, g1_deg=14,
                  power_iters=40, safety=1.5, dense_cap=None, precond="g1", precision="fp64", max_batch=1, outer=None,
-                 refine=0, ctx=None):
+                 refine=0, ctx=None, g1_fp64=False):
         self.ctx = ctx
+        self.g1_fp64 = g1_fp64
         self.precision = precision
         self.max_batch = max_batch
         self.refine = refine

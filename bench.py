@@ -442,6 +442,11 @@ def main():
                            refine=a.refine)
     plain32_ms, _, rel_p = batched(H32p)
     H32p.close()
+    # precision A/B: the G_1 preconditioner's inner iteration in double in the FP32 steps too
+    H32g = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
+                           refine=a.refine, g1_fp64=True)
+    g1d_ms, _, rel_g = batched(H32g)
+    H32g.close()
 
     # dominant kernel: the register row kernel of the G_h pass, timed by its own dispatch inside whole applies
     ops = H32.ops_op()
@@ -519,6 +524,8 @@ def main():
         "relerr_fp32_vs_fp64_median": float(np.median(rel)),
         "fp32_plain": {"inverses_per_s": B * 1000.0 / plain32_ms, "relerr_max": float(rel_p.max()),
                        "relerr_median": float(np.median(rel_p))},
+        "fp32_g1_fp64": {"inverses_per_s": B * 1000.0 / g1d_ms, "relerr_max": float(rel_g.max()),
+                         "relerr_median": float(np.median(rel_g))},
         "wall_ms_per_step": wall_ms,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -156,6 +156,8 @@ typedef struct {
                             (the row-major transposes), copied without a host round trip.  2: the same, but the plan
                             borrows them (no copy): the caller keeps them alive until adrt_plan_destroy, and plans of
                             one size share one set of tables */
+    int g1_fp64;        /* ADRT_PREC_F32 / F32_PLAIN: 1 runs the G_1 preconditioner's inner iteration in double in the
+                           FP32 outer steps too (default 0: in float there, its input rounded to float) */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;

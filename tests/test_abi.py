@@ -45,11 +45,23 @@ def test_reference_signatures():
     assert L.SIGNATURES["omp_max_threads"] == (I, [])
 
 
-def test_plan_desc_layout():
-    """adrt_plan_desc field order and size as the C compiler lays it out
-    (6 ints, 3 doubles, 5 pointers, int + pad, 2 pointers, int + pad, 2 pointers, 2 ints)."""
+def test_plan_desc_layout(tmp_path):
+    """The ctypes adrt_plan_desc has every field of the header's struct at the offset the C compiler gives it, and
+    the same size (gcc compiles the header and prints them)."""
+    import subprocess
+
     L = pkg("lib")
-    assert ctypes.sizeof(L.PlanDesc) == 6 * 4 + 3 * 8 + 5 * 8 + 8 + 2 * 8 + 8 + 2 * 8 + 8
+    names = [f[0] for f in L.PlanDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"adrt_hip.h\"\nint main(void) {\n"
+                   '  printf("%zu\\n", sizeof(adrt_plan_desc));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(adrt_plan_desc, {n}));\n' for n in names)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)])
+    got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    assert got[0] == ctypes.sizeof(L.PlanDesc)
+    assert got[1:] == [getattr(L.PlanDesc, n).offset for n in names]
 
 
 def test_every_package_module_imports():

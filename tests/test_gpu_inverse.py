@@ -220,3 +220,25 @@ def test_plan_batch_limits(gpu):
     with pytest.raises(L.HipError, match="max_batch"):
         H.apply_device(db, dx, 3)
     H.apply_device(db, dx, 0)
+
+
+def test_fp32_with_double_g1(gpu):
+    """The FP32 plan with the G_1 inner iteration in double (g1_fp64): within 1e-5 of the FP64 plan, different bits
+    from the float-G_1 plan (the option takes effect), and a sinogram alone equal to itself in the batch."""
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    n, S, cap = 7, 3, 12
+    N = 1 << n
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(70 + i, N)), 0.01, 80 + i) for i in range(S)])
+    H64 = I.FastHybridHip(n, dense_cap=cap, max_batch=S)
+    x64 = H64.apply_batch(B)
+    kw = dict(lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=S, refine=1,
+              outer=(H64.lo, H64.hi, H64.outer_deg))
+    Hd = I.FastHybridHip(n, g1_fp64=True, **kw)
+    Hf = I.FastHybridHip(n, **kw)
+    xd, xf = Hd.apply_batch(B), Hf.apply_batch(B)
+    for i in range(S):
+        assert np.linalg.norm(xd[i] - x64[i]) / np.linalg.norm(x64[i]) < 1e-5, i
+    assert not np.array_equal(xd, xf)
+    assert np.array_equal(Hd.apply(B[1]), xd[1])
