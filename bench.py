@@ -511,7 +511,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 rows (TwoSum-compensated) + f64 dense GEMM/precond/outer/refine",
+        "dtype": "f32 offset FFTs, rows (explicit level 1 included; TwoSum-compensated Chebyshev) and G1 inner "
+                 "iteration; f64 dense table GEMM, outer iteration and residual, backprojection and last refine pass",
         "data": "synthetic: ellipse phantoms (seeds 1000+i) + 1% Gaussian noise (seeds 2000+i)",
         "config": {"workload": f"hybrid inverse N={N}, batch {B}, dense_cap={cap}, eps_in=0.1, tau=0.2N, g1_deg=14",
                    "n": n, "batch": B, "dense_cap": cap, "outer_deg": H64.outer_deg, "fp64_refine_steps": a.refine,
