@@ -108,8 +108,9 @@ void timing_events(Plan* p, RowsArgs& A) {
 }
 
 int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t st, void* out,
-              const c128* WT = nullptr) {
+              const c128* WT = nullptr, bool io64 = false) {
     RowsArgs A{};
+    A.io64 = io64 ? 1 : 0;
     A.Y = p->Y;
     A.Z = out;
     A.rows = (long long)S * 2 * p->L;
@@ -159,8 +160,10 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
             ADRT_TRY(launch_rows(A, prec, st, &dense_done));
         }
         if (dense_done) return OK;
-        if (mfma) return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
-        return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, prec, st);
+        // the dense GEMM's row type: complex128 when the row kernel took complex128 rows
+        const int rp = io64 ? PREC_F64 : prec;
+        if (mfma) return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, rp, st);
+        return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, rp, st);
     }
     timing_events(p, A);
     return launch_rows(A, prec, st);
@@ -168,14 +171,16 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
 
 // One packed pass: strips -> FFT -> row ops -> iFFT -> unpack -> combine (fastops.py:209-220).
 // mode 0: src is an image batch, mode 1: a sinogram batch.  If resid_y != null, out = resid_y - pass(src).
+// l1_64 (FP32 passes): the FFTs, the combine and the explicit level 1 in double around the float rows.
 int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops, int nops, const double* resid_y,
-                void* out, int S, hipStream_t st, bool out_f32 = false) {
+                void* out, int S, hipStream_t st, bool out_f32 = false, bool l1_64 = false) {
     const int n = p->d.n;
-    const void* tw = prec == PREC_F64 ? p->tw : p->tw32;
-    ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, prec, tw, st));
-    ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, p->Y2, mode == 1 ? p->WbackT : p->WopT));
-    ADRT_TRY(launch_ifft_unpack(n, p->Y2, p->Sq, S, prec, tw, st));
-    return launch_combine(n, p->Sq, prec, resid_y, out, S, st, out_f32);
+    const int pf = l1_64 ? PREC_F64 : prec;  // precision of the FFTs and the combine
+    const void* tw = pf == PREC_F64 ? p->tw : p->tw32;
+    ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, pf, tw, st));
+    ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, p->Y2, mode == 1 ? p->WbackT : p->WopT, l1_64));
+    ADRT_TRY(launch_ifft_unpack(n, p->Y2, p->Sq, S, pf, tw, st));
+    return launch_combine(n, p->Sq, pf, resid_y, out, S, st, out_f32);
 }
 
 // P (the outer preconditioner): g1_inv by fixed-degree Chebyshev on [4, 16], lorr, or the identity.
@@ -258,6 +263,9 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     ADRT_REQUIRE(d->refine >= 0, "plan_create: refine must be >= 0");
     ADRT_REQUIRE(d->dense_on_device >= 0 && d->dense_on_device <= 2, "plan_create: dense_on_device must be 0, 1 or 2");
     ADRT_REQUIRE(d->g1_fp64 == 0 || d->g1_fp64 == 1, "plan_create: g1_fp64 must be 0 or 1");
+    ADRT_REQUIRE(d->l1_fp64 == 0 || d->l1_fp64 == 1, "plan_create: l1_fp64 must be 0 or 1");
+    ADRT_REQUIRE(!d->l1_fp64 || (d->precision == ADRT_PREC_F32 && d->kind == ADRT_KIND_HYBRID && d->n >= 5),
+                 "plan_create: l1_fp64 needs an FP32 (compensated) hybrid plan with n >= 5");
     ADRT_REQUIRE(d->z && ((d->deg && d->a && d->c) || d->kind == ADRT_KIND_NORMAL),
                  "plan_create: null row parameter array");
     ADRT_REQUIRE(d->kind != ADRT_KIND_NORMAL || d->n_dense == 0, "plan_create: the normal operator has no dense rows");
@@ -659,7 +667,9 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     // refinement: the FP64 steps pull the iterate to the FP64 operator's fixed point)
     const int prec = p->d.precision;
     const int refine = adrt_plan_refine_steps(plan);
-    ADRT_TRY(packed_pass(p, refine > 0 ? PREC_F64 : prec, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st));
+    const int pb = refine > 0 ? PREC_F64 : prec;
+    ADRT_TRY(packed_pass(p, pb, 1, d_b, p->ops_back, p->nback, nullptr, p->y, S, st, false,
+                         p->d.l1_fp64 && pb == PREC_F32));
     // iterate x_j lands in buffer j % 2, with buffer (D - 1) % 2 == d_x
     double* X[2];
     X[(D - 1) % 2] = d_x;
@@ -688,7 +698,8 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
         const int pg = p->d.g1_fp64 ? PREC_F64 : pk;
         // an FP32 step's residual feeds only the float G_1 iteration, which rounds it to float: store it as float
         const bool res32 = fused && pg == PREC_F32;
-        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st, res32));
+        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, p->y, p->res, S, st, res32,
+                             p->d.l1_fp64 && pk == PREC_F32));
         if (fused) {
             ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, res32 ? nullptr : p->res, xk, xp, a, om, om - 1.0,
                                      X[k % 2], S, st, pg, res32 ? reinterpret_cast<const float*>(p->res) : nullptr));

@@ -336,6 +336,20 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
     ADRT_REQUIRE(A.dn.n_dense == 0 || (A.skip_dense && A.Z != A.Y && dense_done),
                  "rows: dense tiles need skip_dense, distinct input and output rows and a completion flag");
     const int cls = A.cfg_class >= 0 ? A.cfg_class : prec;
+    if (A.io64) {
+        // complex128 rows with a float core: the register kernel only (the generic kernel's rows are the compute type)
+        ADRT_REQUIRE(A.loge >= 0, "rows: complex128 rows with a float core need the register kernel");
+        const int cfg = rows_config_get(A.n, cls);
+        const int loge = A.loge > 0 ? A.loge : (cfg > 0 ? cfg : default_loge(A.n, prec));
+        ADRT_REQUIRE(loge > 0, "rows: complex128 rows with a float core need 32 <= N <= 2048");
+        const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, prec) >= 0;
+        RowsArgs R = A;
+        if (!fuse) R.dn = DenseArgs{};
+        const int st = launch_rows_reg(R, prec, loge, s);
+        if (st == E_INVALID) set_error("rows: the register kernel does not take this op list or size with complex128 rows");
+        if (st == OK && dense_done) *dense_done = fuse;
+        return st;
+    }
     if (A.loge >= 0 && rows_config_get(A.n, cls) >= 0) {
         const int cfg = rows_config_get(A.n, cls);
         const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));

@@ -42,6 +42,9 @@ struct RowsArgs {
     // dispatch's timestamps, not markers that may be processed while the previous kernel still runs)
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int comp = 0;  // FP32 only: compensated (TwoSum) accumulation of the Chebyshev iterate
+    // FP32 register kernel only: the rows in Y and Z are complex128, and a leading B 1..1 / trailing BT 1..1 of the
+    // op list (the hybrid's explicit level) runs in double on them; the ops between run in float
+    int io64 = 0;
     // Chebyshev step weights (w_j a, w_j - 1) of parameter row k at wcoef[wcoff[k] + j], 1 <= j < deg[k]
     // (cheb_table_fill); the register kernel takes op lists with P ops only when they are given
     const c128* wcoef = nullptr;

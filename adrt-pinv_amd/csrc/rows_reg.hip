@@ -1,5 +1,5 @@
 // Host side of the register row kernel (rows_reg.hpp): op-list decoding, default mapping, dispatch to the
-// instantiation units rows_reg_{d,s,c}{1,2}.hip.
+// instantiation units rows_reg_{d,s,c}{1,2}.hip and rows_reg_x2.hip.
 #include "rows_reg.hpp"
 
 #include <algorithm>
@@ -157,6 +157,11 @@ int launch_rows_reg(const RowsArgs& A, int prec, int loge, hipStream_t s) {
     RA.nk = A.nk > 0 ? A.nk : A.period;
     if (A.skip_dense && !A.group_by_k) return E_INVALID;  // only the frequency-grouped mapping can skip dense rows
     const bool lo1 = RA.lo == 1;
+    if (A.io64) {
+        ADRT_REQUIRE(prec == PREC_F32 && A.comp && !lo1,
+                     "rows: complex128 rows with a float core need the compensated FP32 kernel and levels 2..n");
+        return rows_reg_launch_x2(RA, loge, s);
+    }
     if (prec == PREC_F64) return lo1 ? rows_reg_launch_d1(RA, loge, s) : rows_reg_launch_d2(RA, loge, s);
     if (A.comp) return lo1 ? rows_reg_launch_c1(RA, loge, s) : rows_reg_launch_c2(RA, loge, s);
     return lo1 ? rows_reg_launch_s1(RA, loge, s) : rows_reg_launch_s2(RA, loge, s);

@@ -28,6 +28,8 @@
 // share of the 16 x 16 MFMA tiles (dense.hpp), while the row blocks behind them fill the remaining wave slots.  The tiles' loads and matrix-core work then overlap the
 // rows' vector work instead of running as a launch of their own after it.
 #pragma once
+#include <type_traits>
+
 #include "rows.hpp"
 
 namespace adrt {
@@ -54,6 +56,8 @@ int rows_reg_launch_s1(const RegArgs& RA, int loge, hipStream_t s);
 int rows_reg_launch_s2(const RegArgs& RA, int loge, hipStream_t s);
 int rows_reg_launch_c1(const RegArgs& RA, int loge, hipStream_t s);
 int rows_reg_launch_c2(const RegArgs& RA, int loge, hipStream_t s);
+// float with TwoSum, levels 2..n, complex128 rows in and out with the explicit level 1 in double (RowsArgs::io64)
+int rows_reg_launch_x2(const RegArgs& RA, int loge, hipStream_t s);
 
 namespace rreg {
 
@@ -391,7 +395,7 @@ __device__ __forceinline__ void fwd_up(cplx<T> (&v)[1 << LOGE], cplx<T>* X, cplx
     }
 }
 
-template <typename T, int LOGN, int LOGE, bool COMP, int LO>
+template <typename T, int LOGN, int LOGE, bool COMP, int LO, bool X64>
 __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW)) void rows_reg_kernel(RegArgs RA) {
     using C = Cfg<T, LOGN, LOGE>;
     static_assert(C::cross_on_top(), "the buffer alternation of transition() needs the cross-wavefront edges on top");
@@ -399,6 +403,9 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     constexpr int E = C::E;
     constexpr int NP = C::NPASS;
     constexpr int LAST = NP - 1;
+    // global row type: the compute type, or complex128 with the explicit level in double (X64)
+    using TR = std::conditional_t<X64, double, T>;
+    static_assert(!X64 || LO == 2, "complex128 rows with a float core: the hybrid's op lists (explicit level 1)");
     const RowsArgs& A = RA.r;
     __shared__ cplx<T> Xs[C::NBUF * C::RPB * C::ROW];
     __shared__ int sdeg[C::RPB];
@@ -409,7 +416,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     if (A.dn.workers > 0) {
         if (bid < A.dn.dblk) {
             const int w = xcd_block(bid, A.dn.dblk) * (C::NT / 64) + (int)(threadIdx.x >> 6);
-            if (w < A.dn.workers) dense::worker<T, 4>(LOGN, A.dn, w, threadIdx.x & 63, A.Y, A.Z);
+            if (w < A.dn.workers) dense::worker<TR, 4>(LOGN, A.dn, w, threadIdx.x & 63, A.Y, A.Z);
             return;
         }
         bid -= A.dn.dblk;
@@ -505,12 +512,26 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     // natural (= grouping 0 physical) index of the lane's element 0; element u is x0 + u
     const int x0 = C::phys_c(0, lane, 0);
     static_assert(C::phys_c(0, 0, E - 1) == E - 1, "grouping 0 holds E consecutive values per lane");
-    // (the global row type is T: rows are stored in the compute precision)
-    const cplx<T>* Yr = reinterpret_cast<const cplx<T>*>(A.Y) + row * A.in_stride;
+    const cplx<TR>* Yr = reinterpret_cast<const cplx<TR>*>(A.Y) + row * A.in_stride;
+    int o0 = 0, o1 = RA.nmops;
+    if constexpr (X64) {
+        // level 1 pairs natural neighbours (2l, 2l + 1), which sit in one lane here, with twiddles (1, z):
+        // (f0 + f1, f0 + z f1) in double from the complex128 row, rounded once to T (fastops.py:330)
+        const bool l1 = RA.mop[0] == MOP_L1F;
 #pragma unroll
-    for (int u = 0; u < E; ++u) v[u] = valid ? Yr[x0 + u] : mk((T)0, (T)0);
+        for (int u = 0; u < E; u += 2) {
+            const c128 f0 = valid ? Yr[x0 + u] : mk(0.0, 0.0), f1 = valid ? Yr[x0 + u + 1] : mk(0.0, 0.0);
+            v[u] = cast<T>(l1 ? add(f0, f1) : f0);
+            v[u + 1] = cast<T>(l1 ? add(f0, mul(zk, f1)) : f1);
+        }
+        o0 = l1 ? 1 : 0;
+        if (o1 > o0 && RA.mop[o1 - 1] == MOP_L1A) --o1;
+    } else {
+#pragma unroll
+        for (int u = 0; u < E; ++u) v[u] = valid ? Yr[x0 + u] : mk((T)0, (T)0);
+    }
 
-    for (int o = 0; o < RA.nmops; ++o) {
+    for (int o = o0; o < o1; ++o) {
         const int op = RA.mop[o];
         if (op == MOP_L1F) {
             pass_fwd<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
@@ -637,13 +658,24 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
         }
     }
     if (valid) {
-        cplx<T>* Zr = reinterpret_cast<cplx<T>*>(A.Z) + row * A.out_stride;
+        cplx<TR>* Zr = reinterpret_cast<cplx<TR>*>(A.Z) + row * A.out_stride;
+        if constexpr (X64) {
+            // the adjoint of level 1, (g0 + g1, g0 + conj(z) g1), in double on the float results (fastops.py:330)
+            const bool l1 = o1 < RA.nmops;
 #pragma unroll
-        for (int u = 0; u < E; ++u) Zr[x0 + u] = v[u];
+            for (int u = 0; u < E; u += 2) {
+                const c128 g0 = cast<double>(v[u]), g1 = cast<double>(v[u + 1]);
+                Zr[x0 + u] = l1 ? add(g0, g1) : g0;
+                Zr[x0 + u + 1] = l1 ? add(g0, mul(conj(zk), g1)) : g1;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < E; ++u) Zr[x0 + u] = v[u];
+        }
     }
 }
 
-template <typename T, int LOGN, int LOGE, bool COMP, int LO>
+template <typename T, int LOGN, int LOGE, bool COMP, int LO, bool X64>
 int launch_cfg(const RegArgs& RA, hipStream_t s) {
     using C = Cfg<T, LOGN, LOGE>;
     long long nblk;
@@ -667,38 +699,38 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         return E_INVALID;
     }
     if (RA.r.ev_start)
-        hipExtLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s,
+        hipExtLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO, X64>), dim3((unsigned)nblk), dim3(C::NT), 0, s,
                               RA.r.ev_start, RA.r.ev_stop, 0, RB);
     else
-        hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RB);
+        hipLaunchKernelGGL((rows_reg_kernel<T, LOGN, LOGE, COMP, LO, X64>), dim3((unsigned)nblk), dim3(C::NT), 0, s, RB);
     return launch_check("rows_reg_kernel");
 }
 
-template <typename T, int LOGN, bool COMP, int LO>
+template <typename T, int LOGN, bool COMP, int LO, bool X64>
 int launch_n(const RegArgs& RA, int loge, hipStream_t s) {
     if constexpr (LOGN >= 2) {
-        if (loge == 2) return launch_cfg<T, LOGN, 2, COMP, LO>(RA, s);
+        if (loge == 2) return launch_cfg<T, LOGN, 2, COMP, LO, X64>(RA, s);
     }
     if constexpr (LOGN >= 3) {
-        if (loge == 3) return launch_cfg<T, LOGN, 3, COMP, LO>(RA, s);
+        if (loge == 3) return launch_cfg<T, LOGN, 3, COMP, LO, X64>(RA, s);
     }
     if constexpr (LOGN >= 4) {
-        if (loge == 4) return launch_cfg<T, LOGN, 4, COMP, LO>(RA, s);
+        if (loge == 4) return launch_cfg<T, LOGN, 4, COMP, LO, X64>(RA, s);
     }
     set_error("rows_reg: unsupported elements-per-lane");
     return E_INVALID;
 }
 
-template <typename T, bool COMP, int LO>
+template <typename T, bool COMP, int LO, bool X64 = false>
 int launch_t(const RegArgs& RA, int loge, hipStream_t s) {
     switch (RA.r.n) {
-        case 5: return launch_n<T, 5, COMP, LO>(RA, loge, s);
-        case 6: return launch_n<T, 6, COMP, LO>(RA, loge, s);
-        case 7: return launch_n<T, 7, COMP, LO>(RA, loge, s);
-        case 8: return launch_n<T, 8, COMP, LO>(RA, loge, s);
-        case 9: return launch_n<T, 9, COMP, LO>(RA, loge, s);
-        case 10: return launch_n<T, 10, COMP, LO>(RA, loge, s);
-        case 11: return launch_n<T, 11, COMP, LO>(RA, loge, s);
+        case 5: return launch_n<T, 5, COMP, LO, X64>(RA, loge, s);
+        case 6: return launch_n<T, 6, COMP, LO, X64>(RA, loge, s);
+        case 7: return launch_n<T, 7, COMP, LO, X64>(RA, loge, s);
+        case 8: return launch_n<T, 8, COMP, LO, X64>(RA, loge, s);
+        case 9: return launch_n<T, 9, COMP, LO, X64>(RA, loge, s);
+        case 10: return launch_n<T, 10, COMP, LO, X64>(RA, loge, s);
+        case 11: return launch_n<T, 11, COMP, LO, X64>(RA, loge, s);
         default: set_error("rows_reg: n must be 5..11"); return E_INVALID;
     }
 }

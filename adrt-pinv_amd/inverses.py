@@ -15,6 +15,8 @@ eigensolve) touches the host.  Extra keywords:
   refine     with precision 'fp32': the backprojection and the last ``refine`` outer steps run in FP64
   g1_fp64    (hybrid) with precision 'fp32': the G_1 preconditioner's inner iteration runs in double in the FP32
              outer steps too (default: in float there)
+  l1_fp64    (hybrid) with precision 'fp32': the explicit level 1 and the offset FFTs and combine around it run in
+             double in the FP32 outer steps (default: in float); the rows' levels 2..n and Chebyshev filter stay float
   ctx        a lib.Context whose device-memory budget the plan is charged to
 """
 
@@ -82,6 +84,7 @@ class _PlanBase:
         d.refine = int(self.refine)
         d.dense_on_device = 2
         d.g1_fp64 = int(bool(getattr(self, "g1_fp64", False)))
+        d.l1_fp64 = int(bool(getattr(self, "l1_fp64", False)))
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         ctx = getattr(self, "ctx", None)
@@ -239,9 +242,10 @@ class FastHybridHip(_PlanBase):
 
     def __init__(self, n, lam_bound="power", tau_fac=0.2, eps_in=0.1, eps=1e-10, threads=8, lam_max=None, g1_deg=14,
                  power_iters=40, safety=1.5, dense_cap=None, precond="g1", precision="fp64", max_batch=1, outer=None,
-                 refine=0, ctx=None, g1_fp64=False):
+                 refine=0, ctx=None, g1_fp64=False, l1_fp64=False):
         self.ctx = ctx
         self.g1_fp64 = g1_fp64
+        self.l1_fp64 = l1_fp64
         self.precision = precision
         self.max_batch = max_batch
         self.refine = refine

@@ -16,7 +16,7 @@ import numpy as np
 # written by an older build are not reused.
 VERSION = 2
 # keywords that do not change the setup scalars of an FP64 plan
-_NOT_SETUP = ("max_batch", "precision", "refine", "ctx", "outer", "lam_max", "g1_fp64")
+_NOT_SETUP = ("max_batch", "precision", "refine", "ctx", "outer", "lam_max", "g1_fp64", "l1_fp64")
 
 
 def cache_dir():

@@ -447,6 +447,17 @@ def main():
                            refine=a.refine, g1_fp64=True)
     g1d_ms, _, rel_g = batched(H32g)
     H32g.close()
+    # precision A/B: the explicit level 1 with the FFTs and combine around it in double, and that together with G_1
+    # in double (FP64 for the explicit cross-quadrant level, the table GEMM and the residual)
+    abl = {}
+    for key, kw in (("fp32_l1_fp64", dict(l1_fp64=True)), ("fp32_l1_g1_fp64", dict(l1_fp64=True, g1_fp64=True))):
+        Hx = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
+                             refine=a.refine, **kw)
+        ms_x, _, rel_x = batched(Hx)
+        Hx.close()
+        abl[key] = {"inverses_per_s": B * 1000.0 / ms_x, "relerr_max": float(rel_x.max()),
+                    "relerr_median": float(np.median(rel_x))}
+        log(f"{key}: {abl[key]['inverses_per_s']:.0f} inverses/s, relerr max {rel_x.max():.2e}")
 
     # dominant kernel: the register row kernel of the G_h pass, timed by its own dispatch inside whole applies
     ops = H32.ops_op()
@@ -527,6 +538,7 @@ def main():
                        "relerr_median": float(np.median(rel_p))},
         "fp32_g1_fp64": {"inverses_per_s": B * 1000.0 / g1d_ms, "relerr_max": float(rel_g.max()),
                          "relerr_median": float(np.median(rel_g))},
+        **abl,
         "wall_ms_per_step": wall_ms,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -158,6 +158,9 @@ typedef struct {
                             one size share one set of tables */
     int g1_fp64;        /* ADRT_PREC_F32 / F32_PLAIN: 1 runs the G_1 preconditioner's inner iteration in double in the
                            FP32 outer steps too (default 0: in float there, its input rounded to float) */
+    int l1_fp64;        /* ADRT_PREC_F32 hybrid plans: 1 runs the explicit level 1 (B 1..1, BT 1..1 of fastops.py:330), the
+                           offset FFTs and the combine of the FP32 outer steps in double; the row kernel then reads and
+                           writes complex128 rows and runs levels 2..n and the Chebyshev filter in float (default 0) */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;

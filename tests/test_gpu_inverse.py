@@ -242,3 +242,60 @@ def test_fp32_with_double_g1(gpu):
         assert np.linalg.norm(xd[i] - x64[i]) / np.linalg.norm(x64[i]) < 1e-5, i
     assert not np.array_equal(xd, xf)
     assert np.array_equal(Hd.apply(B[1]), xd[1])
+
+
+@pytest.mark.parametrize("n,cap,loge,dense", [(7, 12, 0, 0), (8, 16, 2, -1), (8, 16, 4, 2), (6, 8, 3, 1)])
+def test_fp32_with_double_explicit_level(gpu, n, cap, loge, dense):
+    """The FP32 plan with the explicit level 1 (and the FFTs and combine around it) in double (l1_fp64): the row
+    kernel reads and writes complex128 rows, with levels 2..n and the Chebyshev filter in float.  At refine 0 (where the
+    backprojection takes that kernel too) and refine 1: within 1e-5 of the FP64 plan, different bits from the
+    float-level plan, a sinogram alone equal to itself in the batch; under the row mappings E = 4, 8, 16 and with the
+    dense tiles inside the row launch (1, 2 waves per SIMD) or in a launch of their own (complex128 rows there too)."""
+    I = pkg("inverses")
+    L = pkg("lib")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    lib = L.load()
+    S = 3
+    N = 1 << n
+    print(f"l1_fp64 n={n} cap={cap} loge={loge} dense={dense}", flush=True)
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(90 + i, N)), 0.01, 95 + i) for i in range(S)])
+    H64 = I.FastHybridHip(n, dense_cap=cap, max_batch=S)
+    assert H64.eng.n_dense > 0
+    x64 = H64.apply_batch(B)
+    old = lib.adrt_rows_config_get(n, 1), lib.adrt_dense_config_get(n, 1)
+    try:
+        assert lib.adrt_rows_config_set(n, 1, loge) == 0 and lib.adrt_dense_config_set(n, 1, dense) == 0
+        for refine in (0, 1):
+            kw = dict(lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=S, refine=refine,
+                      outer=(H64.lo, H64.hi, H64.outer_deg))
+            Hx = I.FastHybridHip(n, l1_fp64=True, **kw)
+            Hf = I.FastHybridHip(n, **kw)
+            xx, xf = Hx.apply_batch(B), Hf.apply_batch(B)
+            for i in range(S):
+                e = np.linalg.norm(xx[i] - x64[i]) / np.linalg.norm(x64[i])
+                assert e < 1e-5, (refine, i, e)
+            assert not np.array_equal(xx, xf)
+            assert np.array_equal(Hx.apply(B[1]), xx[1])
+            Hx.close()
+            Hf.close()
+    finally:
+        lib.adrt_rows_config_set(n, 1, old[0])
+        lib.adrt_dense_config_set(n, 1, old[1])
+    H64.close()
+
+
+def test_double_explicit_level_needs_fp32_hybrid(gpu):
+    """l1_fp64 is refused for an FP64 plan and an uncompensated FP32 plan; L_p (no explicit level) has no such
+    keyword."""
+    I = pkg("inverses")
+    L = pkg("lib")
+    n = 6
+    H = I.FastHybridHip(n, dense_cap=8)
+    kw = dict(lam_max=H.lam_max, outer=(H.lo, H.hi, H.outer_deg), l1_fp64=True)
+    for extra in (dict(), dict(precision="fp32_plain")):
+        with pytest.raises(L.HipError, match="l1_fp64"):
+            I.FastHybridHip(n, dense_cap=8, **kw, **extra)
+    with pytest.raises(TypeError):
+        I.FastLpHip(n, precision="fp32", l1_fp64=True)
+    H.close()
