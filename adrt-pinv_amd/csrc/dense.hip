@@ -101,52 +101,61 @@ int launch_n(int n, int L, int SP, int n_dense, const int* dense_k, const c128* 
 }
 
 // ---------------------------------------------------------------- MFMA version (FP64 matrix cores)
-// Each wave owns one 16 x 16 output tile (dense.hpp: dense::tile).  A block is four waves: WR = min(4, rows / 16) of
-// them on consecutive row tiles and 4 / WR on consecutive 16-output column tiles, so with many rows the block's W^T
-// columns are fetched once into L1 and shared, and with few rows (a single sinogram) every wave still has rows to
-// work on.  At N = 256 with 128 rows and 33 dense frequencies that is 4224 waves of equal work, about four per SIMD,
-// so the tail costs one wave's time.  Blocks are numbered so that the blocks of one frequency run on one XCD (the
+// Each wave owns a block of RT x CT 16 x 16 output tiles (dense.hpp: dense::tile).  A block is four waves: WR of them
+// on consecutive row blocks and 4 / WR on consecutive output blocks (WR = 4 with many rows, fewer with few), so with
+// many rows the block's W^T columns are fetched once into L1 and shared, and with few rows (a single sinogram) every
+// wave still has rows to work on.  Blocks are numbered so that the blocks of one frequency run on one XCD (the
 // dispatcher deals consecutive blocks to the 8 XCDs in turn): the frequency's W^T then comes from HBM once per XCD
 // and from that XCD's L2 for its other row groups.
-template <typename Tg, int U>
-__global__ __launch_bounds__(256) void dense_mfma_kernel(int n, DenseArgs D, const void* Yin, void* Zout, int gx,
-                                                         int gy, int nblk, int wr) {
+// tiles per wave of the stand-alone kernel (rows, outputs).  Measured at N = 256 / 512 with 128 / 92 rows per
+// frequency (us per launch): 1 x 1 with 8 k-steps per load group 67 / 398; 1 x 2 62 / 343; 2 x 1 73 / 380; 2 x 2 79 / 352;
+// 4 x 1 88 / 474.  The four waves of a block already share W through L1, so row blocking only costs registers.
+constexpr int SA_RT = 1, SA_CT = 2;
+
+template <typename Tg, int U, int RT, int CT>
+__global__ __launch_bounds__(256, 2) void dense_mfma_kernel(int n, DenseArgs D, const void* Yin, void* Zout, int gx,
+                                                            int gy, int nblk, int wr) {
     const int N = 1 << n;
     const int lb = xcd_block(blockIdx.x, nblk);
     if (lb < 0) return;  // (the kernel has no barrier)
     const int bx = lb % gx, by = (lb / gx) % gy, slot = lb / (gx * gy);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = (by * wr + wave % wr) * 16;
-    const int c0 = (bx * (4 / wr) + wave / wr) * 16;
+    const int r0 = (by * wr + wave % wr) * 16 * RT;
+    const int c0 = (bx * (4 / wr) + wave / wr) * 16 * CT;
     if (r0 >= D.SP || c0 >= N) return;
-    dense::tile<Tg, U>(n, D, slot, r0, c0, lane, Yin, Zout);
+    dense::tile<Tg, U, RT, CT>(n, D, slot, r0, c0, lane, Yin, Zout);
+}
+
+template <typename Tg, int U, int RT, int CT>
+int launch_mfma_rc(int n, const DenseArgs& D, const void* Y, void* Z, hipStream_t st) {
+    const int N = 1 << n;
+    const int rb = (D.SP + 16 * RT - 1) / (16 * RT);    // row blocks
+    const int cb = (N / 16 + CT - 1) / CT;              // output blocks
+    const int wr = rb >= 4 ? 4 : (rb >= 2 ? 2 : 1);     // waves of a block along the rows
+    const int gx = (cb + 4 / wr - 1) / (4 / wr), gy = (rb + wr - 1) / wr;
+    const long long nblk = (long long)gx * gy * D.n_dense;
+    ADRT_REQUIRE(nblk <= 0x7fffffffLL, "dense_mfma: too many blocks");
+    hipLaunchKernelGGL((dense_mfma_kernel<Tg, U, RT, CT>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, D, Y, Z, gx, gy,
+                       (int)nblk, wr);
+    return launch_check("dense_mfma_kernel");
 }
 
 template <typename Tg>
 int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
                 hipStream_t st) {
-    const int N = 1 << n;
-    const int wr = SP > 32 ? 4 : (SP > 16 ? 2 : 1);  // waves of a block along the rows
-    const int gx = (N / 16 + 4 / wr - 1) / (4 / wr), gy = (SP + 16 * wr - 1) / (16 * wr);
-    const long long nblk = (long long)gx * gy * n_dense;
-    ADRT_REQUIRE(nblk <= 0x7fffffffLL, "dense_mfma: too many blocks");
     DenseArgs D;
     D.dense_k = dense_k;
     D.WT = WT;
     D.n_dense = n_dense;
     D.L = L;
     D.SP = SP;
-    // 8 k-steps per load group for complex64 rows; complex128 rows take 4 (the larger group's registers would cost
-    // half the waves)
-    if (N >= 32 && sizeof(Tg) == 4)
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 8>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, D, Y, Z, gx, gy,
-                           (int)nblk, wr);
-    else
-        hipLaunchKernelGGL((dense_mfma_kernel<Tg, 4>), dim3(xcd_grid(nblk)), dim3(256), 0, st, n, D, Y, Z, gx, gy,
-                           (int)nblk, wr);
-    return launch_check("dense_mfma_kernel");
+    // blocked waves beyond one row tile; a single row tile (one or a few sinograms, where W streams from HBM) keeps
+    // 16 x 16 tiles, twice the waves, with deeper load groups (8 k-steps for complex64 rows; complex128 rows take 4:
+    // the larger group's registers would cost half the waves)
+    if (SP > 16) return launch_mfma_rc<Tg, 4, SA_RT, SA_CT>(n, D, Y, Z, st);
+    if ((1 << n) >= 32 && sizeof(Tg) == 4) return launch_mfma_rc<Tg, 8, 1, 1>(n, D, Y, Z, st);
+    return launch_mfma_rc<Tg, 4, 1, 1>(n, D, Y, Z, st);
 }
-
 }  // namespace
 
 // Z_r = W_slot Y_r on the FP64 matrix cores (requires N >= 16 and Z distinct from Y).

@@ -21,83 +21,142 @@ struct DenseArgs {
     int SP = 0;
     int workers = 0;  // register row kernel: waves that take dense tiles (0 = none)
     int dblk = 0;     // register row kernel: leading blocks that hold those waves (a multiple of 8)
+    int rt = 1, ct = 1;  // register row kernel: row and output tiles per work unit of a dense wave
 };
 
 namespace dense {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// The tile of rows r0..r0+15 and outputs c0..c0+15 of dense slot `slot`, computed by one wave (lane 0..63).  U
-// k-steps (of 4) per load group, the next group's loads in flight while the current group's 3 U MFMAs run.  Zout
-// must not alias Yin (other waves still read the frequency's rows).
-template <typename Tg, int U>
+// The block of RT x CT tiles (rows r0 .. r0 + 16 RT - 1, outputs c0 .. c0 + 16 CT - 1) of dense slot `slot`, computed
+// by one wave (lane 0..63): each k-step loads RT row fragments and CT W fragments and runs 3 RT CT MFMAs, so a W
+// fragment serves RT row tiles and a row fragment CT output tiles.  U k-steps (of 4) per load group, the next group's
+// loads in flight while the current group's MFMAs run.  Every output element accumulates the same k-steps in the same
+// order for any RT, CT and U, so the bits do not depend on the blocking.  Zout must not alias Yin (other waves still
+// read the frequency's rows).
+template <typename Tg, int U, int RT, int CT>
 __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0, int c0, int lane, const void* Yin,
                                      void* Zout) {
     const int N = 1 << n;
     const int k = D.dense_k[slot];
     const int li = lane & 15, kq = lane >> 4;
-    const c128* W = D.WT + (size_t)slot * N * N + c0 + li;  // B[kq][li] of k-step kb: W[(kb + kq) * N]
-    const int row = r0 + li;
-    const bool rv = row < D.SP;
-    const cplx<Tg>* Arow = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv ? row : 0) * D.L + k) * N + kq;
+    const c128* W = D.WT + (size_t)slot * N * N + c0 + li;  // B[kq][li] of k-step kb, tile ct: W[(kb + kq) * N + 16 ct]
+    const cplx<Tg>* Arow[RT];
+    bool rv[RT], cv[CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int row = r0 + 16 * rt + li;
+        rv[rt] = row < D.SP;
+        Arow[rt] = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv[rt] ? row : 0) * D.L + k) * N + kq;
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) cv[ct] = c0 + 16 * ct < N;
     const cplx<Tg> zero = mk((Tg)0, (Tg)0);
-    v4d p1 = v4d{0.0, 0.0, 0.0, 0.0}, p2 = v4d{0.0, 0.0, 0.0, 0.0}, p3 = v4d{0.0, 0.0, 0.0, 0.0};
-    cplx<Tg> a[U], an[U];
-    c128 b[U], bn[U];
+    const c128 zero64 = mk(0.0, 0.0);
+    v4d p1[RT][CT], p2[RT][CT], p3[RT][CT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) p1[rt][ct] = p2[rt][ct] = p3[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
+    cplx<Tg> a[U][RT], an[U][RT];
+    c128 b[U][CT], bn[U][CT];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        a[u] = rv ? Arow[4 * u] : zero;
-        b[u] = W[(size_t)(4 * u + kq) * N];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) a[u][rt] = rv[rt] ? Arow[rt][4 * u] : zero;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) b[u][ct] = cv[ct] ? W[(size_t)(4 * u + kq) * N + 16 * ct] : zero64;
     }
     for (int kb = 0; kb < N; kb += 4 * U) {
         const int kn = kb + 4 * U;
         if (kn < N) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                an[u] = rv ? Arow[kn + 4 * u] : zero;
-                bn[u] = W[(size_t)(kn + 4 * u + kq) * N];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) an[u][rt] = rv[rt] ? Arow[rt][kn + 4 * u] : zero;
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    bn[u][ct] = cv[ct] ? W[(size_t)(kn + 4 * u + kq) * N + 16 * ct] : zero64;
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const double ar = (double)a[u].x, ai = (double)a[u].y;
-            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[u].x, p1, 0, 0, 0);
-            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[u].y, p2, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, b[u].x + b[u].y, p3, 0, 0, 0);
+            double ar[RT], ai[RT], as[RT], bs[CT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                ar[rt] = (double)a[u][rt].x;
+                ai[rt] = (double)a[u][rt].y;
+                as[rt] = ar[rt] + ai[rt];
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) bs[ct] = b[u][ct].x + b[u][ct].y;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    p1[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], b[u][ct].x, p1[rt][ct], 0, 0, 0);
+                    p2[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], b[u][ct].y, p2[rt][ct], 0, 0, 0);
+                    p3[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[rt], bs[ct], p3[rt][ct], 0, 0, 0);
+                }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            a[u] = an[u];
-            b[u] = bn[u];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) a[u][rt] = an[u][rt];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) b[u][ct] = bn[u][ct];
         }
     }
     cplx<Tg>* Z = reinterpret_cast<cplx<Tg>*>(Zout);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int sp = r0 + 4 * e + kq;
-        if (sp < D.SP) Z[((size_t)sp * D.L + k) * N + c0 + li] = cast<Tg>(mk(p1[e] - p2[e], (p3[e] - p1[e]) - p2[e]));
-    }
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int sp = r0 + 16 * rt + 4 * e + kq;
+                const v4d& q1 = p1[rt][ct];
+                const v4d& q2 = p2[rt][ct];
+                const v4d& q3 = p3[rt][ct];
+                if (sp < D.SP && cv[ct])
+                    Z[((size_t)sp * D.L + k) * N + c0 + 16 * ct + li] =
+                        cast<Tg>(mk(q1[e] - q2[e], (q3[e] - q1[e]) - q2[e]));
+            }
 }
 
-// Tiles of a pass: slot-major, then 16-output column tile, then 16-row tile, so consecutive tiles of a worker share
-// W columns and a frequency's tiles sit on few workers.
+// Work units of a pass: blocks of RT x CT tiles, slot-major, then output block, then row block, so consecutive units
+// of a worker share W columns and a frequency's units sit on few workers.
 __host__ __device__ inline int row_tiles(const DenseArgs& D) { return (D.SP + 15) / 16; }
-__host__ __device__ inline long long tiles(int n, const DenseArgs& D) {
-    return (long long)D.n_dense * ((1 << n) / 16) * row_tiles(D);
+__host__ __device__ inline long long units(int n, const DenseArgs& D, int RT, int CT) {
+    return (long long)D.n_dense * (((1 << n) / 16 + CT - 1) / CT) * ((row_tiles(D) + RT - 1) / RT);
 }
 
-// Dense worker w of D.workers: its contiguous share of the pass's tiles, one after another.
-template <typename Tg, int U>
-__device__ __forceinline__ void worker(int n, const DenseArgs& D, int w, int lane, const void* Yin, void* Zout) {
-    const long long T = tiles(n, D);
+__host__ __device__ inline long long tiles(int n, const DenseArgs& D) { return units(n, D, D.rt, D.ct); }
+
+// Dense worker w of D.workers: its contiguous share of the pass's units of RT x CT tiles, one after another.
+template <typename Tg, int U, int RT, int CT>
+__device__ __forceinline__ void worker_b(int n, const DenseArgs& D, int w, int lane, const void* Yin, void* Zout) {
+    const long long T = units(n, D, RT, CT);
     const long long t0 = T * w / D.workers, t1 = T * (w + 1) / D.workers;
-    const int gr = row_tiles(D), gc = (1 << n) / 16;
+    const int gr = (row_tiles(D) + RT - 1) / RT, gc = ((1 << n) / 16 + CT - 1) / CT;
     for (long long t = t0; t < t1; ++t) {
         const int r = (int)(t % gr);
         const int c = (int)((t / gr) % gc);
         const int slot = (int)(t / ((long long)gr * gc));
-        tile<Tg, U>(n, D, slot, 16 * r, 16 * c, lane, Yin, Zout);
+        tile<Tg, U, RT, CT>(n, D, slot, 16 * RT * r, 16 * CT * c, lane, Yin, Zout);
     }
+}
+
+// BR x BC: the blocking the calling kernel's register budget allows; the host sets (D.rt, D.ct) to it or to 1 x 1.
+template <typename Tg, int U, int BR, int BC>
+__device__ __forceinline__ void worker(int n, const DenseArgs& D, int w, int lane, const void* Yin, void* Zout) {
+    if constexpr (BR * BC > 1) {
+        if (D.rt == BR && D.ct == BC) {
+            worker_b<Tg, U, BR, BC>(n, D, w, lane, Yin, Zout);
+            return;
+        }
+    }
+    worker_b<Tg, U, 1, 1>(n, D, w, lane, Yin, Zout);
 }
 
 }  // namespace dense

@@ -130,6 +130,10 @@ struct Cfg {
     static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN >= 7)   ? 4
                                 : (sizeof(T) == 4 && LOGE == 3 && LOGN <= 9) ? 3
                                                                              : 1;
+    // Tiles per unit of the dense waves in the same launch (rows x outputs): 1 x 2 (each row fragment serves both
+    // output tiles) where the rows leave the registers for it (3 waves per SIMD); at 4 waves the blocked tile would
+    // spill, and the kernels without an occupancy bound would lose a wave to it.
+    static constexpr int DENSE_BR = 1, DENSE_BC = MINW == 3 ? 2 : 1;
     __host__ __device__ static constexpr int kb(int K) { return (LOGN - K * LOGE) < LOGE ? (LOGN - K * LOGE) : LOGE; }
     using LY = Layout<LOGN, LOGE>;
     // LDS entry of physical index p (linear in p's bits)
@@ -416,7 +420,7 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     if (A.dn.workers > 0) {
         if (bid < A.dn.dblk) {
             const int w = xcd_block(bid, A.dn.dblk) * (C::NT / 64) + (int)(threadIdx.x >> 6);
-            if (w < A.dn.workers) dense::worker<TR, 4>(LOGN, A.dn, w, threadIdx.x & 63, A.Y, A.Z);
+            if (w < A.dn.workers) dense::worker<TR, 4, C::DENSE_BR, C::DENSE_BC>(LOGN, A.dn, w, threadIdx.x & 63, A.Y, A.Z);
             return;
         }
         bid -= A.dn.dblk;
@@ -686,7 +690,10 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
     RegArgs RB = RA;
     if (RA.r.dn.n_dense > 0) {
         // dense workers in leading blocks of C::NT / 64 waves, padded to a multiple of the XCD count
-        RB.r.dn.workers = dense_workers(LOGN, sizeof(T) == 8 ? PREC_F64 : PREC_F32, RA.r.dn);
+        const bool blk = dense::row_tiles(RA.r.dn) >= C::DENSE_BR && (1 << LOGN) / 16 >= C::DENSE_BC;
+        RB.r.dn.rt = blk ? C::DENSE_BR : 1;
+        RB.r.dn.ct = blk ? C::DENSE_BC : 1;
+        RB.r.dn.workers = dense_workers(LOGN, sizeof(T) == 8 ? PREC_F64 : PREC_F32, RB.r.dn);
         const int wpb = C::NT / 64;
         RB.r.dn.dblk = (int)xcd_grid((RB.r.dn.workers + wpb - 1) / wpb);
         nblk += RB.r.dn.dblk;

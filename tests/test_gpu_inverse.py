@@ -180,16 +180,19 @@ def test_fp32_batch_headline_config(gpu):
     assert np.array_equal(x32[2], H32.apply(B[2]))
 
 
-@pytest.mark.parametrize("prec,n", [("fp32", 8), ("fp64", 8), ("fp32", 9)])
-def test_dense_placement_same_bits(gpu, prec, n):
+@pytest.mark.parametrize("prec,n,S", [("fp32", 8, 3), ("fp64", 8, 3), ("fp32", 9, 3), ("fp32", 8, 24),
+                                       ("fp64", 8, 12), ("fp32", 7, 9)])
+def test_dense_placement_same_bits(gpu, prec, n, S):
     """The dense low-frequency GEMM gives the same bits whichever launch computes it (a launch of its own, or 1, 2, 4
-    MFMA waves per SIMD inside the row kernel's launch): every placement runs the same tile arithmetic in the same
-    order.  Batch 3 puts FP64 passes on the MFMA path too (6 rows > 4)."""
+    MFMA waves per SIMD inside the row kernel's launch) and whatever its blocking: every placement runs the same tile
+    arithmetic in the same order.  Batch 3 puts FP64 passes on the MFMA path too (6 rows > 4) with one row tile per
+    frequency; batches 9, 12 and 24 (18, 24, 48 rows) take the blocked tiles (two row tiles per dense wave inside the
+    row launch, 2 x 2 tiles per wave in a launch of its own), with a partly filled last block."""
     I = pkg("inverses")
     ph = pkg("phantom")
     R = pkg("radon")
     lib = gpu.load()
-    S, cap = 3, 16
+    cap = 12 if n == 7 else 16
     N = 1 << n
     B = np.stack([ph.add_noise(R.adrt(ph.ellipses(300 + i, N)), 0.01, 400 + i) for i in range(S)])
     H = I.FastHybridHip(n, dense_cap=cap, precision=prec, max_batch=S, refine=1)
