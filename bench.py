@@ -376,6 +376,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ablations", action="store_true",
+                    help="skip the precision ablations (no TwoSum, G_1 / explicit level in double)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", default="200,500,1000,2000,5000",
                     help="Poisson arrival rates (req/s), comma separated")
@@ -463,22 +465,17 @@ def main():
     ips = B * 1000.0 / step_ms
     log(f"batched fp32 {step_ms:.2f} ms per step of {B}: {ips:.0f} inverses/s, relerr max {rel.max():.2e}")
 
-    # ablation: the same batch without the TwoSum compensation
-    H32p = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32_plain", max_batch=B, outer=outer,
-                           refine=a.refine)
-    plain32_ms, _, rel_p = batched(H32p)
-    H32p.close()
-    # precision A/B: the G_1 preconditioner's inner iteration in double in the FP32 steps too
-    H32g = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
-                           refine=a.refine, g1_fp64=True)
-    g1d_ms, _, rel_g = batched(H32g)
-    H32g.close()
-    # precision A/B: the explicit level 1 with the FFTs and combine around it in double, and that together with G_1
-    # in double (FP64 for the explicit cross-quadrant level, the table GEMM and the residual)
+    # precision ablations on the same batch (each its own plan and graph), skipped with --no-ablations
     abl = {}
-    for key, kw in (("fp32_l1_fp64", dict(l1_fp64=True)), ("fp32_l1_g1_fp64", dict(l1_fp64=True, g1_fp64=True))):
-        Hx = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, precision="fp32", max_batch=B, outer=outer,
-                             refine=a.refine, **kw)
+    variants = (("fp32_plain", dict(precision="fp32_plain")),  # without the TwoSum compensation
+                ("fp32_g1_fp64", dict(g1_fp64=True)),  # the G_1 inner iteration in double in the FP32 steps too
+                # the explicit level 1 with the FFTs and combine around it in double, and that together with G_1 in
+                # double (FP64 for the explicit cross-quadrant level, the table GEMM and the residual)
+                ("fp32_l1_fp64", dict(l1_fp64=True)),
+                ("fp32_l1_g1_fp64", dict(l1_fp64=True, g1_fp64=True)))
+    for key, kw in (() if a.no_ablations else variants):
+        kw = {"precision": "fp32", **kw}
+        Hx = I.FastHybridHip(n, lam_max=H64.lam_max, dense_cap=cap, max_batch=B, outer=outer, refine=a.refine, **kw)
         ms_x, _, rel_x = batched(Hx)
         Hx.close()
         abl[key] = {"inverses_per_s": B * 1000.0 / ms_x, "relerr_max": float(rel_x.max()),
@@ -560,11 +557,7 @@ def main():
         "speedup_vs_plain_fp64": ips / plain_ips,
         "relerr_fp32_vs_fp64_max": float(rel.max()),
         "relerr_fp32_vs_fp64_median": float(np.median(rel)),
-        "fp32_plain": {"inverses_per_s": B * 1000.0 / plain32_ms, "relerr_max": float(rel_p.max()),
-                       "relerr_median": float(np.median(rel_p))},
-        "fp32_g1_fp64": {"inverses_per_s": B * 1000.0 / g1d_ms, "relerr_max": float(rel_g.max()),
-                         "relerr_median": float(np.median(rel_g))},
-        **abl,
+        **{k: abl.get(k) for k, _ in variants},
         "wall_ms_per_step": wall_ms,
         "roofline": roof,
         "cpu_baseline": cpu,
