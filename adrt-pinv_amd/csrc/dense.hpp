@@ -41,18 +41,22 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
     const int k = D.dense_k[slot];
     const int li = lane & 15, kq = lane >> 4;
     const c128* W = D.WT + (size_t)slot * N * N + c0 + li;  // B[kq][li] of k-step kb, tile ct: W[(kb + kq) * N + 16 ct]
+    // A row past the frequency's rows, or an output tile past N, only feeds outputs that are never stored (row i of
+    // the product depends on row i of A alone, column j on column j of W alone), so such lanes and tiles read a valid
+    // clamped address (row 0, tile 0) instead of selecting zeros: no predicated or flat loads in the k loop.
     const cplx<Tg>* Arow[RT];
-    bool rv[RT], cv[CT];
+    bool cv[CT];
+    int coff[CT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
         const int row = r0 + 16 * rt + li;
-        rv[rt] = row < D.SP;
-        Arow[rt] = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(rv[rt] ? row : 0) * D.L + k) * N + kq;
+        Arow[rt] = reinterpret_cast<const cplx<Tg>*>(Yin) + ((size_t)(row < D.SP ? row : 0) * D.L + k) * N + kq;
     }
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) cv[ct] = c0 + 16 * ct < N;
-    const cplx<Tg> zero = mk((Tg)0, (Tg)0);
-    const c128 zero64 = mk(0.0, 0.0);
+    for (int ct = 0; ct < CT; ++ct) {
+        cv[ct] = c0 + 16 * ct < N;
+        coff[ct] = cv[ct] ? 16 * ct : 0;
+    }
     v4d p1[RT][CT], p2[RT][CT], p3[RT][CT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -63,9 +67,9 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) a[u][rt] = rv[rt] ? Arow[rt][4 * u] : zero;
+        for (int rt = 0; rt < RT; ++rt) a[u][rt] = Arow[rt][4 * u];
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct) b[u][ct] = cv[ct] ? W[(size_t)(4 * u + kq) * N + 16 * ct] : zero64;
+        for (int ct = 0; ct < CT; ++ct) b[u][ct] = W[(size_t)(4 * u + kq) * N + coff[ct]];
     }
     for (int kb = 0; kb < N; kb += 4 * U) {
         const int kn = kb + 4 * U;
@@ -73,10 +77,9 @@ __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0
 #pragma unroll
             for (int u = 0; u < U; ++u) {
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) an[u][rt] = rv[rt] ? Arow[rt][kn + 4 * u] : zero;
+                for (int rt = 0; rt < RT; ++rt) an[u][rt] = Arow[rt][kn + 4 * u];
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct)
-                    bn[u][ct] = cv[ct] ? W[(size_t)(kn + 4 * u + kq) * N + 16 * ct] : zero64;
+                for (int ct = 0; ct < CT; ++ct) bn[u][ct] = W[(size_t)(kn + 4 * u + kq) * N + coff[ct]];
             }
         }
 #pragma unroll
@@ -137,13 +140,19 @@ __host__ __device__ inline long long tiles(int n, const DenseArgs& D) { return u
 template <typename Tg, int U, int RT, int CT>
 __device__ __forceinline__ void worker_b(int n, const DenseArgs& D, int w, int lane, const void* Yin, void* Zout) {
     const long long T = units(n, D, RT, CT);
-    const long long t0 = T * w / D.workers, t1 = T * (w + 1) / D.workers;
+    const int t0 = (int)(T * w / D.workers), t1 = (int)(T * (w + 1) / D.workers);
     const int gr = (row_tiles(D) + RT - 1) / RT, gc = ((1 << n) / 16 + CT - 1) / CT;
-    for (long long t = t0; t < t1; ++t) {
-        const int r = (int)(t % gr);
-        const int c = (int)((t / gr) % gc);
-        const int slot = (int)(t / ((long long)gr * gc));
+    // unit t = (slot, output block c, row block r), r fastest; divided once, then stepped
+    int r = t0 % gr, c = (t0 / gr) % gc, slot = t0 / (gr * gc);
+    for (int t = t0; t < t1; ++t) {
         tile<Tg, U, RT, CT>(n, D, slot, 16 * RT * r, 16 * CT * c, lane, Yin, Zout);
+        if (++r == gr) {
+            r = 0;
+            if (++c == gc) {
+                c = 0;
+                ++slot;
+            }
+        }
     }
 }
 

@@ -234,7 +234,9 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
 #pragma unroll
         for (int e = 0; e < BPT; ++e) {
             const int b = threadIdx.x + G1_THREADS * e;
-            if (b >= active) continue;
+            // a cell outside the image keeps x = 0 (its LDS entries stay as cleared); N and the region origin are
+            // even, so a 2 x 2 cell is inside or outside as a whole
+            if (b >= active || !inm[e]) continue;
             const int o = lds[e];
             const T up0 = Xr[2 * BUF + o - CP], up1 = Xr[3 * BUF + o - CP];  // x[r0 - 1][c0], x[r0 - 1][c0 + 1]
             const T dn0 = Xr[o + CP], dn1 = Xr[BUF + o + CP];                // x[r0 + 2][c0], x[r0 + 2][c0 + 1]
@@ -261,15 +263,13 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const bool in = (inm[e] >> c) & 1;
                 if constexpr (INCR) {
-                    // a pixel outside the image has y = 0 and x = 0 from the start: masking its increment keeps it 0
-                    const T d = in ? om * g[c] + om1 * xq[e][c] : (T)0;
+                    const T d = om * g[c] + om1 * xq[e][c];
                     xq[e][c] = d;
                     xc[e][c] = xc[e][c] + d;
                 } else {
                     const T xo = xc[e][c];
-                    const T xn = in ? xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]) : (T)0;
+                    const T xn = xo + om * (a * (yv[e][c] - g[c])) + om1 * (xo - xq[e][c]);
                     xq[e][c] = xo;
                     xc[e][c] = xn;
                 }

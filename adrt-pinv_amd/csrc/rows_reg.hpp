@@ -693,6 +693,10 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         const bool blk = dense::row_tiles(RA.r.dn) >= C::DENSE_BR && (1 << LOGN) / 16 >= C::DENSE_BC;
         RB.r.dn.rt = blk ? C::DENSE_BR : 1;
         RB.r.dn.ct = blk ? C::DENSE_BC : 1;
+        if (dense::tiles(LOGN, RB.r.dn) > 0x7fffffffLL) {  // the workers step through units with int counters
+            set_error("rows_reg: too many dense tiles");
+            return E_INVALID;
+        }
         RB.r.dn.workers = dense_workers(LOGN, sizeof(T) == 8 ? PREC_F64 : PREC_F32, RB.r.dn);
         const int wpb = C::NT / 64;
         RB.r.dn.dblk = (int)xcd_grid((RB.r.dn.workers + wpb - 1) / wpb);
