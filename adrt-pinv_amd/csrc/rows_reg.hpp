@@ -46,6 +46,7 @@ struct RegArgs {
     int SP;              // rows per frequency when group_by_k
     const int* korder;   // frequency order of the blocks (may be null)
     int nk;              // frequencies scheduled (korder entries)
+    int iter = 1;        // group_by_k: row groups (of RPB rows) a block runs one after another (set by the launcher)
 };
 
 // Launchers of one compute type and lower level, rows_reg_<type><lo>.hip (d: double, s: float, c: float with the
@@ -432,23 +433,25 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     cplx<T>* X = Xs + rib * C::ROW;
     cplx<T>* X2 = C::NBUF > 1 ? X + C::RPB * C::ROW : nullptr;  // second exchange buffer (cross-wavefront edges)
 
-    // ---- which row
-    long long row;
-    bool valid;
+    // ---- which rows.  With the frequency-grouped mapping a block runs the row groups sg0 .. sg1 - 1 (RA.iter groups
+    // of RPB rows) of one frequency k one after another, so the per-frequency set-up below (the z^t table and the
+    // twiddle pairs in registers) is done once for all of them; otherwise a block has one row per row slot.
+    int k, sg0 = 0, sg1 = 1;
+    long long row1 = 0;
+    bool valid1 = true;
     if (RA.group_by_k) {
         const int nspb = (RA.SP + C::RPB - 1) / C::RPB;
-        const int kk = bid / nspb;
-        const int sg = bid % nspb;
-        const int k = RA.korder ? RA.korder[kk] : kk;
-        const int sp = sg * C::RPB + rib;
-        valid = sp < RA.SP;
-        row = (long long)(valid ? sp : 0) * A.period + k;
+        const int nbk = (nspb + RA.iter - 1) / RA.iter;  // blocks per frequency
+        const int kk = bid / nbk;
+        sg0 = (bid % nbk) * RA.iter;
+        sg1 = min(sg0 + RA.iter, nspb);
+        k = RA.korder ? RA.korder[kk] : kk;
     } else {
-        row = (long long)bid * C::RPB + rib;
-        valid = row < A.rows;
-        if (!valid) row = 0;
+        row1 = (long long)bid * C::RPB + rib;
+        valid1 = row1 < A.rows;
+        if (!valid1) row1 = 0;
+        k = (int)(row1 % A.period);
     }
-    const int k = (int)(row % A.period);
     const c128 zk = A.z[k];
 
     // ---- table of z^t, t = 0 .. N/2, in the row's LDS buffer: copied from the plan's per-frequency table when there
@@ -511,183 +514,211 @@ __global__ __launch_bounds__((Cfg<T, LOGN, LOGE>::NT), (Cfg<T, LOGN, LOGE>::MINW
     };
     __syncthreads();
 
-    // ---- load the row in natural order (grouping 0: lane holds E consecutive values)
-    cplx<T> v[E];
     // natural (= grouping 0 physical) index of the lane's element 0; element u is x0 + u
     const int x0 = C::phys_c(0, lane, 0);
     static_assert(C::phys_c(0, 0, E - 1) == E - 1, "grouping 0 holds E consecutive values per lane");
-    const cplx<TR>* Yr = reinterpret_cast<const cplx<TR>*>(A.Y) + row * A.in_stride;
-    int o0 = 0, o1 = RA.nmops;
-    if constexpr (X64) {
-        // level 1 pairs natural neighbours (2l, 2l + 1), which sit in one lane here, with twiddles (1, z):
-        // (f0 + f1, f0 + z f1) in double from the complex128 row, rounded once to T (fastops.py:330)
-        const bool l1 = RA.mop[0] == MOP_L1F;
-#pragma unroll
-        for (int u = 0; u < E; u += 2) {
-            const c128 f0 = valid ? Yr[x0 + u] : mk(0.0, 0.0), f1 = valid ? Yr[x0 + u + 1] : mk(0.0, 0.0);
-            v[u] = cast<T>(l1 ? add(f0, f1) : f0);
-            v[u + 1] = cast<T>(l1 ? add(f0, mul(zk, f1)) : f1);
+    for (int sg = sg0; sg < sg1; ++sg) {  // block-uniform trip count: the ops' barriers stay uniform
+        // the previous row's last exchange may still be read by other wavefronts, and a leading BT op writes the whole
+        // buffer: a barrier between rows
+        if (sg != sg0) __syncthreads();
+        long long row = row1;
+        bool valid = valid1;
+        if (RA.group_by_k) {
+            const int sp = sg * C::RPB + rib;
+            valid = sp < RA.SP;
+            row = (long long)(valid ? sp : 0) * A.period + k;
         }
-        o0 = l1 ? 1 : 0;
-        if (o1 > o0 && RA.mop[o1 - 1] == MOP_L1A) --o1;
-    } else {
+        // ---- load the row in natural order (grouping 0: lane holds E consecutive values)
+        cplx<T> v[E];
+        const cplx<TR>* Yr = reinterpret_cast<const cplx<TR>*>(A.Y) + row * A.in_stride;
+        int o0 = 0, o1 = RA.nmops;
+        if constexpr (X64) {
+            // level 1 pairs natural neighbours (2l, 2l + 1), which sit in one lane here, with twiddles (1, z):
+            // (f0 + f1, f0 + z f1) in double from the complex128 row, rounded once to T (fastops.py:330)
+            const bool l1 = RA.mop[0] == MOP_L1F;
 #pragma unroll
-        for (int u = 0; u < E; ++u) v[u] = valid ? Yr[x0 + u] : mk((T)0, (T)0);
-    }
+            for (int u = 0; u < E; u += 2) {
+                const c128 f0 = valid ? Yr[x0 + u] : mk(0.0, 0.0), f1 = valid ? Yr[x0 + u + 1] : mk(0.0, 0.0);
+                v[u] = cast<T>(l1 ? add(f0, f1) : f0);
+                v[u + 1] = cast<T>(l1 ? add(f0, mul(zk, f1)) : f1);
+            }
+            o0 = l1 ? 1 : 0;
+            if (o1 > o0 && RA.mop[o1 - 1] == MOP_L1A) --o1;
+        } else {
+#pragma unroll
+            for (int u = 0; u < E; ++u) v[u] = valid ? Yr[x0 + u] : mk((T)0, (T)0);
+        }
 
-    for (int o = o0; o < o1; ++o) {
-        const int op = RA.mop[o];
-        if (op == MOP_L1F) {
-            pass_fwd<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
-        } else if (op == MOP_L1A) {
-            pass_adj<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
-        } else if (op == MOP_GRAM) {
-            gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);
+        for (int o = o0; o < o1; ++o) {
+            const int op = RA.mop[o];
+            if (op == MOP_L1F) {
+                pass_fwd<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
+            } else if (op == MOP_L1A) {
+                pass_adj<T, LOGN, LOGE, 0, 1, 1>(v, tw[0], tz[0]);
+            } else if (op == MOP_GRAM) {
+                gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = scale((T)2, v[u]);  // exact
-        } else if (op == MOP_BTNAT) {
-            // natural -> physical order of levels 1..n through LDS, then the adjoint passes down to grouping 0
+                for (int u = 0; u < E; ++u) v[u] = scale((T)2, v[u]);  // exact
+            } else if (op == MOP_BTNAT) {
+                // natural -> physical order of levels 1..n through LDS, then the adjoint passes down to grouping 0
 #pragma unroll
-            for (int u = 0; u < E; ++u) X[C::addr(lam(x0 + u, LOGN))] = v[u];
-            __syncthreads();
+                for (int u = 0; u < E; ++u) X[C::addr(lam(x0 + u, LOGN))] = v[u];
+                __syncthreads();
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = X[C::template base<LAST>(lane) + C::template off<LAST>(u)];
-            __syncthreads();
-            cplx<T> tzf[NP][E];
-            pairs_full(tzf);
-            adj_down<T, LOGN, LOGE, LAST, LO>(v, X, X2, tw, tzf, lane);
-        } else if (op == MOP_BEND) {
-            cplx<T> tzf[NP][E];
-            pairs_full(tzf);
-            fwd_up<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tzf, lane);
+                for (int u = 0; u < E; ++u) v[u] = X[C::template base<LAST>(lane) + C::template off<LAST>(u)];
+                __syncthreads();
+                cplx<T> tzf[NP][E];
+                pairs_full(tzf);
+                adj_down<T, LOGN, LOGE, LAST, LO>(v, X, X2, tw, tzf, lane);
+            } else if (op == MOP_BEND) {
+                cplx<T> tzf[NP][E];
+                pairs_full(tzf);
+                fwd_up<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tzf, lane);
 #pragma unroll
-            for (int u = 0; u < E; ++u) X[C::template base<LAST>(lane) + C::template off<LAST>(u)] = v[u];
-            __syncthreads();
+                for (int u = 0; u < E; ++u) X[C::template base<LAST>(lane) + C::template off<LAST>(u)] = v[u];
+                __syncthreads();
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = X[C::addr(lam(x0 + u, LOGN))];
-            __syncthreads();
-        } else {  // MOP_P
-            const int slot = A.dslot ? A.dslot[k] : -1;
-            const bool dense = valid && slot >= 0;
-            const int d = (valid && !dense) ? A.deg[k] : 0;
-            if (lane == 0) sdeg[rib] = d * 2 + (dense ? 1 : 0);
-            __syncthreads();
-            int dmax = 0, anyd = 0;
+                for (int u = 0; u < E; ++u) v[u] = X[C::addr(lam(x0 + u, LOGN))];
+                __syncthreads();
+            } else {  // MOP_P
+                const int slot = A.dslot ? A.dslot[k] : -1;
+                const bool dense = valid && slot >= 0;
+                const int d = (valid && !dense) ? A.deg[k] : 0;
+                if (lane == 0) sdeg[rib] = d * 2 + (dense ? 1 : 0);
+                __syncthreads();
+                int dmax = 0, anyd = 0;
 #pragma unroll
-            for (int r = 0; r < C::RPB; ++r) {
-                dmax = max(dmax, sdeg[r] >> 1);
-                anyd |= sdeg[r] & 1;
-            }
-            __syncthreads();
-            // The iterate is zi; with COMP it is the unevaluated sum zi + zl, and the recurrence carries its increment
-            // dl = z_j - z_(j-1) instead of the previous iterate: z_(j+1) = z_j + dl_j with dl_j = (w a) r_j + (w - 1)
-            // dl_(j-1) (the reference's z + w (a r) + (w - 1)(z - z_prev), cpp/oblique.cpp:93-97), each increment
-            // added to zi by TwoSum and its rounding error to zl.  dl starts at z_0 - 0 (the zero start).  The
-            // residual uses zi alone: tau zl is below the rounding error of the FP32 product M z.
-            constexpr int EL = COMP ? E : 1;
-            constexpr int EP = COMP ? 1 : E;
-            cplx<T> y[E], zi[E], zp[EP], zl[EL], dl[EL];
-            const double ak = valid ? A.a[k] : 0.0;
-            const T aT = (T)ak, tau = (T)A.tau;
-            // the step weights come from a table (no double division in the loop)
-            const c128* wc = A.wcoef + A.wcoff[k];
-#pragma unroll
-            for (int u = 0; u < E; ++u) {
-                y[u] = v[u];
-                zi[u] = scale(aT, y[u]);
-                if constexpr (COMP) {
-                    zl[u] = mk(fma(aT, y[u].x, -zi[u].x), fma(aT, y[u].y, -zi[u].y));
-                    dl[u] = zi[u];
-                } else {
-                    zp[u] = mk((T)0, (T)0);
+                for (int r = 0; r < C::RPB; ++r) {
+                    dmax = max(dmax, sdeg[r] >> 1);
+                    anyd |= sdeg[r] & 1;
                 }
-            }
-            for (int it = 1; it < dmax; ++it) {
-                const c128 wq = wc[it < d ? it : 0];  // loaded before the Gram: its latency hides behind it
+                __syncthreads();
+                // The iterate is zi; with COMP it is the unevaluated sum zi + zl, and the recurrence carries its
+                // increment dl = z_j - z_(j-1) instead of the previous iterate: z_(j+1) = z_j + dl_j with
+                // dl_j = (w a) r_j + (w - 1) dl_(j-1) (the reference's z + w (a r) + (w - 1)(z - z_prev),
+                // cpp/oblique.cpp:93-97), each increment added to zi by TwoSum and its rounding error to zl.  dl starts
+                // at z_0 - 0 (the zero start).  The residual uses zi alone: tau zl is below the rounding error of the
+                // FP32 product M z.
+                constexpr int EL = COMP ? E : 1;
+                constexpr int EP = COMP ? 1 : E;
+                cplx<T> y[E], zi[E], zp[EP], zl[EL], dl[EL];
+                const double ak = valid ? A.a[k] : 0.0;
+                const T aT = (T)ak, tau = (T)A.tau;
+                // the step weights come from a table (no double division in the loop)
+                const c128* wc = A.wcoef + A.wcoff[k];
 #pragma unroll
-                for (int u = 0; u < E; ++u) v[u] = zi[u];
-                gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);  // v = M z / 2
-                if (it < d) {
-                    // z <- z + (w a) r + (w - 1)(z - z_prev), r = y - M z - tau z, y - M z as one rounding of y - 2 v
-                    const T wa = (T)wq.x, w0 = (T)wq.y;
+                for (int u = 0; u < E; ++u) {
+                    y[u] = v[u];
+                    zi[u] = scale(aT, y[u]);
+                    if constexpr (COMP) {
+                        zl[u] = mk(fma(aT, y[u].x, -zi[u].x), fma(aT, y[u].y, -zi[u].y));
+                        dl[u] = zi[u];
+                    } else {
+                        zp[u] = mk((T)0, (T)0);
+                    }
+                }
+                for (int it = 1; it < dmax; ++it) {
+                    const c128 wq = wc[it < d ? it : 0];  // loaded before the Gram: its latency hides behind it
 #pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        if constexpr (COMP) {
-                            const cplx<T> r = sub(mk(fma((T)-2, v[u].x, y[u].x), fma((T)-2, v[u].y, y[u].y)), scale(tau, zi[u]));
-                            dl[u] = add(scale(wa, r), scale(w0, dl[u]));
-                            cplx<T> hi, err;
-                            two_sum(zi[u], dl[u], hi, err);
-                            zi[u] = hi;
-                            zl[u] = add(zl[u], err);
-                        } else {
-                            const cplx<T> r = sub(mk(fma((T)-2, v[u].x, y[u].x), fma((T)-2, v[u].y, y[u].y)), scale(tau, zi[u]));
-                            const cplx<T> zn = add(add(zi[u], scale(wa, r)), scale(w0, sub(zi[u], zp[u])));
-                            zp[u] = zi[u];
-                            zi[u] = zn;
+                    for (int u = 0; u < E; ++u) v[u] = zi[u];
+                    gram_half<T, LOGN, LOGE, 0, LO>(v, X, X2, tw, tz, hh, lane);  // v = M z / 2
+                    if (it < d) {
+                        // z <- z + (w a) r + (w - 1)(z - z_prev), r = y - M z - tau z, y - M z as one rounding of
+                        // y - 2 v
+                        const T wa = (T)wq.x, w0 = (T)wq.y;
+#pragma unroll
+                        for (int u = 0; u < E; ++u) {
+                            if constexpr (COMP) {
+                                const cplx<T> r = sub(mk(fma((T)-2, v[u].x, y[u].x), fma((T)-2, v[u].y, y[u].y)),
+                                                      scale(tau, zi[u]));
+                                dl[u] = add(scale(wa, r), scale(w0, dl[u]));
+                                cplx<T> hi, err;
+                                two_sum(zi[u], dl[u], hi, err);
+                                zi[u] = hi;
+                                zl[u] = add(zl[u], err);
+                            } else {
+                                const cplx<T> r = sub(mk(fma((T)-2, v[u].x, y[u].x), fma((T)-2, v[u].y, y[u].y)),
+                                                      scale(tau, zi[u]));
+                                const cplx<T> zn = add(add(zi[u], scale(wa, r)), scale(w0, sub(zi[u], zp[u])));
+                                zp[u] = zi[u];
+                                zi[u] = zn;
+                            }
                         }
                     }
                 }
-            }
-            if (anyd) {
-                // exact Tikhonov filter z = F y in double for dense rows (F^T stored: coalesced over the lane's E)
+                if (anyd) {
+                    // exact Tikhonov filter z = F y in double for dense rows (F^T stored: coalesced over the lane's E)
 #pragma unroll
-                for (int u = 0; u < E; ++u) X[C::addr(x0 + u)] = y[u];
-                __syncthreads();
-                if (dense) {
-                    const c128* F = A.dFT + (long long)slot * C::N * C::N;
-                    c128 acc[E];
+                    for (int u = 0; u < E; ++u) X[C::addr(x0 + u)] = y[u];
+                    __syncthreads();
+                    if (dense) {
+                        const c128* F = A.dFT + (long long)slot * C::N * C::N;
+                        c128 acc[E];
 #pragma unroll
-                    for (int u = 0; u < E; ++u) acc[u] = mk(0.0, 0.0);
-                    for (int q = 0; q < C::N; ++q) {
-                        const c128 yq = cast<double>(X[C::addr(q)]);
-                        const c128* Fq = F + (long long)q * C::N + x0;
+                        for (int u = 0; u < E; ++u) acc[u] = mk(0.0, 0.0);
+                        for (int q = 0; q < C::N; ++q) {
+                            const c128 yq = cast<double>(X[C::addr(q)]);
+                            const c128* Fq = F + (long long)q * C::N + x0;
 #pragma unroll
-                        for (int u = 0; u < E; ++u) acc[u] = add(acc[u], mul(Fq[u], yq));
+                            for (int u = 0; u < E; ++u) acc[u] = add(acc[u], mul(Fq[u], yq));
+                        }
+#pragma unroll
+                        for (int u = 0; u < E; ++u) {
+                            zi[u] = cast<T>(acc[u]);
+                            if constexpr (COMP) zl[u] = cast<T>(sub(acc[u], cast<double>(zi[u])));
+                        }
                     }
-#pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        zi[u] = cast<T>(acc[u]);
-                        if constexpr (COMP) zl[u] = cast<T>(sub(acc[u], cast<double>(zi[u])));
-                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
 #pragma unroll
-            for (int u = 0; u < E; ++u) {
-                if constexpr (COMP)
-                    v[u] = add(zi[u], zl[u]);
-                else
-                    v[u] = zi[u];
+                for (int u = 0; u < E; ++u) {
+                    if constexpr (COMP)
+                        v[u] = add(zi[u], zl[u]);
+                    else
+                        v[u] = zi[u];
+                }
             }
         }
-    }
-    if (valid) {
-        cplx<TR>* Zr = reinterpret_cast<cplx<TR>*>(A.Z) + row * A.out_stride;
-        if constexpr (X64) {
-            // the adjoint of level 1, (g0 + g1, g0 + conj(z) g1), in double on the float results (fastops.py:330)
-            const bool l1 = o1 < RA.nmops;
+        if (valid) {
+            cplx<TR>* Zr = reinterpret_cast<cplx<TR>*>(A.Z) + row * A.out_stride;
+            if constexpr (X64) {
+                // the adjoint of level 1, (g0 + g1, g0 + conj(z) g1), in double on the float results (fastops.py:330)
+                const bool l1 = o1 < RA.nmops;
 #pragma unroll
-            for (int u = 0; u < E; u += 2) {
-                const c128 g0 = cast<double>(v[u]), g1 = cast<double>(v[u + 1]);
-                Zr[x0 + u] = l1 ? add(g0, g1) : g0;
-                Zr[x0 + u + 1] = l1 ? add(g0, mul(conj(zk), g1)) : g1;
+                for (int u = 0; u < E; u += 2) {
+                    const c128 g0 = cast<double>(v[u]), g1 = cast<double>(v[u + 1]);
+                    Zr[x0 + u] = l1 ? add(g0, g1) : g0;
+                    Zr[x0 + u + 1] = l1 ? add(g0, mul(conj(zk), g1)) : g1;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < E; ++u) Zr[x0 + u] = v[u];
             }
-        } else {
-#pragma unroll
-            for (int u = 0; u < E; ++u) Zr[x0 + u] = v[u];
         }
-    }
+    }  // row groups
 }
 
 template <typename T, int LOGN, int LOGE, bool COMP, int LO, bool X64>
 int launch_cfg(const RegArgs& RA, hipStream_t s) {
     using C = Cfg<T, LOGN, LOGE>;
-    long long nblk;
-    if (RA.group_by_k)
-        nblk = (long long)RA.nk * ((RA.SP + C::RPB - 1) / C::RPB);
-    else
-        nblk = (RA.r.rows + C::RPB - 1) / C::RPB;
     RegArgs RB = RA;
+    long long nblk;
+    if (RA.group_by_k) {
+        // Float rows: several row groups of a frequency per block share its set-up, as long as the launch keeps at
+        // least 8 wavefronts of rows per SIMD (8 x 1024 on MI355X) to fill the chip and balance its tail (N=1024,
+        // 8 sinograms: 1166 -> 1112 us per launch; double rows measured slower with it, 125 -> 136 us)
+        const int nspb = (RA.SP + C::RPB - 1) / C::RPB;
+        RB.iter = 1;
+        for (int it = sizeof(T) == 4 ? 4 : 1; it > 1; it /= 2)
+            if ((long long)RA.nk * ((nspb + it - 1) / it) * (C::NT / 64) >= 8 * 1024) {
+                RB.iter = it;
+                break;
+            }
+        nblk = (long long)RA.nk * ((nspb + RB.iter - 1) / RB.iter);
+    } else {
+        RB.iter = 1;
+        nblk = (RA.r.rows + C::RPB - 1) / C::RPB;
+    }
     if (RA.r.dn.n_dense > 0) {
         // dense workers in leading blocks of C::NT / 64 waves, padded to a multiple of the XCD count
         const bool blk = dense::row_tiles(RA.r.dn) >= C::DENSE_BR && (1 << LOGN) / 16 >= C::DENSE_BC;
