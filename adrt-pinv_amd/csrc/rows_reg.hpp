@@ -724,7 +724,7 @@ int launch_cfg(const RegArgs& RA, hipStream_t s) {
         const bool blk = dense::row_tiles(RA.r.dn) >= C::DENSE_BR && (1 << LOGN) / 16 >= C::DENSE_BC;
         RB.r.dn.rt = blk ? C::DENSE_BR : 1;
         RB.r.dn.ct = blk ? C::DENSE_BC : 1;
-        if (dense::tiles(LOGN, RB.r.dn) > 0x7fffffffLL) {  // the workers step through units with int counters
+        if (dense::tiles(LOGN, RB.r.dn) > 0x7fff0000LL) {  // the workers step through units with int counters
             set_error("rows_reg: too many dense tiles");
             return E_INVALID;
         }

@@ -29,7 +29,10 @@ sys.path.insert(0, REPO)
 import importlib  # noqa: E402
 
 PEAK = {"fp32": 157.3e12, "fp32_nonpacked": 78.6e12, "fp64": 78.6e12, "hbm": 8.0e12}
-PMC_SUMMARY = os.path.join(REPO, "profiles", "rows_reg_kernel_n256_b64_pmc.txt")
+
+def pmc_summary(N, batch):
+    """The committed rocprofv3 --pmc summary of the row kernel for this workload (N, sinograms per launch)."""
+    return os.path.join(REPO, "profiles", f"rows_reg_kernel_n{N}_b{batch}_pmc.txt")
 
 
 def pkg(sub):
@@ -68,11 +71,11 @@ def rows_kernel_work(H, S, ops):
     return f, nbytes
 
 
-def pmc_traffic():
+def pmc_traffic(N, batch):
     """(HBM bytes per launch of the row kernel, "loge,dense" mapping they were counted at) from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size), or (None, None)."""
     try:
-        txt = open(PMC_SUMMARY).read()
+        txt = open(pmc_summary(N, batch)).read()
     except OSError:
         return None, None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, flags=re.M)
@@ -367,7 +370,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=64)
+    # sinograms per launch of the batched FP32 path.  Measured on one box (inverses/s at N=256): 32: 3355, 64: 3544,
+    # 128: 3668, 256: 3655 -- 128 is where the per-launch costs stop shrinking
+    ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--n", type=int, default=8)
     ap.add_argument("--dense-cap", type=int, default=16)
     ap.add_argument("--refine", type=int, default=1,
@@ -489,7 +494,7 @@ def main():
     fl, nb = rows_kernel_work(H32, B, ops)
     dfl = dense_work(H32, B)[0]
     achieved = fl / (rows_ms * 1e-3)
-    traffic, traffic_map = pmc_traffic()  # counted at the mapping the profile names (row loge, dense placement)
+    traffic, traffic_map = pmc_traffic(H32.N, B)  # counted at the mapping the profile names (row loge, dense placement)
     roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK["fp32"], "traffic": traffic, "traffic_mapping": traffic_map,
             "frac_nonpacked": achieved / PEAK["fp32_nonpacked"],
