@@ -211,6 +211,33 @@ def test_dense_placement_same_bits(gpu, prec, n, S):
         H.close()
 
 
+@pytest.mark.parametrize("n,loge,S", [(10, 3, 8), (10, 2, 8), (8, 3, 64)])
+def test_row_groups_per_block_same_bits(gpu, n, loge, S):
+    """Blocks of float rows run several row groups of their frequency on one set-up once the batch gives the launch
+    enough wavefronts (up to four groups at N=1024 with 8 sinograms, two at N=256 with 64); one sinogram alone runs one
+    group per block.  Every sinogram must give the same bits either way.  refine 0 runs the backprojection in float:
+    its op list starts with the adjoint levels over the whole row, which write the whole exchange buffer, so with rows
+    of two or four wavefronts (N=1024 with 8 or 4 values per lane) a block's rows need the barrier between them."""
+    I = pkg("inverses")
+    C = pkg("plancache")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    lib = gpu.load()
+    N = 1 << n
+    lam, outer = C.hybrid_setup(n, dense_cap=16)
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(700 + i, N)), 0.01, 800 + i) for i in range(S)])
+    old = lib.adrt_rows_config_get(n, 1)
+    H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=16, precision="fp32", max_batch=S, refine=0)
+    try:
+        assert lib.adrt_rows_config_set(n, 1, loge) == 0
+        x = H.apply_batch(B)
+        for i in (0, S // 2, S - 1):
+            assert np.array_equal(x[i], H.apply(B[i])), i
+    finally:
+        lib.adrt_rows_config_set(n, 1, old)
+        H.close()
+
+
 def test_plan_batch_limits(gpu):
     """A batch above the plan's max_batch is an error, not an overrun; an empty batch is a no-op."""
     L = gpu
