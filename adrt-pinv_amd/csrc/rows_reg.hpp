@@ -126,11 +126,13 @@ struct Cfg {
     static constexpr int NPASS = (LOGN + LOGE - 1) / LOGE;
     static constexpr int ROW = N + (N >> LOGE);  // padded row length in LDS (also holds the z^t table, N/2 + 1)
     // Waves per SIMD the register allocation must allow (measured on MI355X): float rows with E = 4 from N = 128 on
-    // run faster at 4 waves even with a few spilled registers than at their natural 3, and E = 8 up to N = 512 at 3
-    // rather than 2; for N >= 1024 with E = 8 the spills cost more than the extra wave.
-    static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN >= 7)   ? 4
-                                : (sizeof(T) == 4 && LOGE == 3 && LOGN <= 9) ? 3
-                                                                             : 1;
+    // run faster at 4 waves even with a few spilled registers than at their natural 3, and E = 8 up to N = 512 and at
+    // N = 2048 at 3 rather than 2 (N = 2048, 4 sinograms: 340 -> 311 ms per batch).  At N = 1024 the extra wave makes
+    // E = 8 about as fast as E = 4 with 8 sinograms (78.6 -> 73.7 against 74.9 ms) but slower with one (32.0 -> 37.8
+    // against 27.0 ms); kept at 2 there, the tuner keeps E = 4, which serves single requests fastest.
+    static constexpr int MINW = (sizeof(T) == 4 && LOGE == 2 && LOGN >= 7)                    ? 4
+                                : (sizeof(T) == 4 && LOGE == 3 && (LOGN <= 9 || LOGN == 11)) ? 3
+                                                                                              : 1;
     // Tiles per unit of the dense waves in the same launch (rows x outputs): 1 x 2 (each row fragment serves both
     // output tiles) where the rows leave the registers for it (3 waves per SIMD); at 4 waves the blocked tile would
     // spill, and the kernels without an occupancy bound would lose a wave to it.
