@@ -102,6 +102,9 @@ struct G1Args {
     double a_out, w1, w0;
     double* out;
     double om[32];     // inner omega_k, k = 1..deg-1
+    int lanes;         // mode 2 with per-image weights (lw1, lw0, lxp) instead of w1, w0 and xp for every image
+    unsigned long long lxp;
+    double lw1[G1_MAX_LANES], lw0[G1_MAX_LANES];
 };
 
 
@@ -168,6 +171,9 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
     const int s = lb / (nt * nt);
     const int gi0 = ((lb / nt) % nt) * TILE - HALO, gj0 = (lb % nt) * TILE - HALO;
     const size_t img = (size_t)s * N * N;
+    // the epilogue's weights and previous iterate: the launch's, or this image's own (images at different steps)
+    const bool hx = A.xp && (!A.lanes || ((A.lxp >> s) & 1ull));
+    const double ew1 = A.lanes ? A.lw1[s] : A.w1, ew0 = A.lanes ? A.lw0[s] : A.w0;
     const T a = (T)(2.0 / 20.0);
     for (int i = threadIdx.x; i < 8 * BUF; i += G1_THREADS) X[i] = (T)0;
     __syncthreads();
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
                 const bool in = b < TB * TB && gi < N && gj < N;
                 const size_t o = (size_t)s * N * N + (size_t)gi * N + gj;
                 xo_[e][c] = in ? A.x[o] : 0.0;
-                xpo_[e][c] = (in && A.xp) ? A.xp[o] : 0.0;
+                xpo_[e][c] = (in && hx) ? A.xp[o] : 0.0;
             }
         }
     }
@@ -297,8 +303,8 @@ __global__ __launch_bounds__(G1_THREADS) void g1_fused_kernel(G1Args A) {
                 A.out[o] = A.a_out * pv;
             } else if (e < EPI) {
                 const double xo = xo_[e][c];
-                const double dx = A.xp ? (xo - xpo_[e][c]) : (xo - 0.0);
-                A.out[o] = xo + A.w1 * (A.a_out * pv) + A.w0 * dx;
+                const double dx = hx ? (xo - xpo_[e][c]) : (xo - 0.0);
+                A.out[o] = xo + ew1 * (A.a_out * pv) + ew0 * dx;
             }
         }
     }
@@ -316,12 +322,13 @@ int launch_g1_cfg(const G1Args& A, int S, hipStream_t st) {
 }
 
 template <typename T>
-int launch_g1_t(const G1Args& A, int S, hipStream_t st) {
+int launch_g1_t(const G1Args& A, int S, int s_tile, hipStream_t st) {
     const int N = 1 << A.n;
     // TILE 64 halves the halo overhead; TILE 32 gives a small batch more blocks.  The float iteration always takes
-    // TILE 64: a pixel's arithmetic then does not depend on the batch size, so a sinogram gives the same bits alone
-    // and inside a batch (in double the tiles differ by at most a rounding of the last bit).
-    const bool big = std::is_same<T, float>::value || (long long)S * ((N + 63) / 64) * ((N + 63) / 64) >= 256;
+    // TILE 64; the double one decides by s_tile, which a plan fixes (its max_batch).  A pixel's arithmetic then does
+    // not depend on how many images share the launch, so a sinogram gives the same bits alone and inside a batch (the
+    // two tile sizes differ by at most a rounding of the last bit).
+    const bool big = std::is_same<T, float>::value || (long long)s_tile * ((N + 63) / 64) * ((N + 63) / 64) >= 256;
     if (A.deg - 1 <= 14) return big ? launch_g1_cfg<T, 64, 14>(A, S, st) : launch_g1_cfg<T, 32, 14>(A, S, st);
     return big ? launch_g1_cfg<T, 64, 16>(A, S, st) : launch_g1_cfg<T, 32, 16>(A, S, st);
 }
@@ -329,8 +336,10 @@ int launch_g1_t(const G1Args& A, int S, hipStream_t st) {
 }  // namespace
 
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
-                    double w1, double w0, double* out, int S, hipStream_t st, int prec, const float* y32) {
+                    double w1, double w0, double* out, int S, hipStream_t st, int prec, const float* y32, int s_tile,
+                    const G1Lanes* lanes) {
     ADRT_REQUIRE(deg >= 1 && deg - 1 <= G1_MAXH, "g1_fused: g1_deg must be 1..17");
+    ADRT_REQUIRE(!lanes || (mode == 2 && S <= G1_MAX_LANES), "g1_fused: per-image weights need mode 2 and S <= 64");
     ADRT_REQUIRE((y != nullptr) != (y32 != nullptr) && (y32 == nullptr || prec == PREC_F32),
                  "g1_fused: give y, or y32 for the float iteration");
     ADRT_REQUIRE(n >= 1 && n <= 13, "g1_fused: need 1 <= n <= 13");
@@ -348,13 +357,22 @@ int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, 
     A.w0 = w0;
     A.out = out;
     A.S = S;
+    if (lanes) {
+        A.lanes = 1;
+        A.lxp = lanes->has_xp;
+        for (int i = 0; i < S; ++i) {
+            A.lw1[i] = lanes->w1[i];
+            A.lw0[i] = lanes->w0[i];
+        }
+    }
     const double c = 12.0 / 20.0;
     double om = 1.0;
     for (int k = 1; k < deg; ++k) {
         om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
         A.om[k - 1] = om;
     }
-    return prec == PREC_F32 ? launch_g1_t<float>(A, S, st) : launch_g1_t<double>(A, S, st);
+    const int stl = s_tile > 0 ? s_tile : S;
+    return prec == PREC_F32 ? launch_g1_t<float>(A, S, stl, st) : launch_g1_t<double>(A, S, stl, st);
 }
 
 int launch_scale(const double* x, double alpha, double* out, long long count, hipStream_t st) {

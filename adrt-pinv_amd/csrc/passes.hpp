@@ -40,13 +40,22 @@ int launch_cheb_update(const double* x, const double* xp, const double* r, doubl
 // means the zero start of the first step.
 int launch_g1_step(int n, const double* y, const double* x, const double* xp, double a, double w1, double w0,
                    double* out, int S, hipStream_t st);
+// Per-image weights of the mode-2 epilogue below, for a batch whose images sit at different outer steps
+// (adrt_plan_tick): image s takes w1[s], w0[s], and its x_prev only where bit s of has_xp is set (else 0).
+constexpr int G1_MAX_LANES = 64;
+struct G1Lanes {
+    double w1[G1_MAX_LANES], w0[G1_MAX_LANES];
+    unsigned long long has_xp = 0;
+};
 // g1_inv(y) (deg Chebyshev steps of G_1 on [4, 16]) in one launch, with an epilogue:
 //   mode 0: out = P y;  mode 1: out = a_out * P y;  mode 2: out = x + w1 * (a_out * P y) + w0 * (x - xp) (xp null = 0)
 // prec PREC_F32 runs the inner iteration in float (outer iterates and the epilogue stay double); its right-hand side
-// may then come as float (y32, with y null).
+// may then come as float (y32, with y null).  s_tile (0: S) is the batch size that picks the double iteration's tile
+// size; a plan passes its max_batch, so that an image's bits do not depend on how many others share the launch.
+// lanes (S <= G1_MAX_LANES) replaces w1, w0 and the use of xp per image.
 int launch_g1_fused(int n, int deg, int mode, const double* y, const double* x, const double* xp, double a_out,
                     double w1, double w0, double* out, int S, hipStream_t st, int prec = PREC_F64,
-                    const float* y32 = nullptr);
+                    const float* y32 = nullptr, int s_tile = 0, const G1Lanes* lanes = nullptr);
 // out = (y - Pi y)/4 + Pi y / 12 (fasthybrid.py:101-111)
 int launch_lorr(int n, const double* y, double* out, int S, hipStream_t st);
 

@@ -193,7 +193,9 @@ int precond(Plan* p, const double* v, double* out, int S, hipStream_t st) {
     }
     if (p->d.precond == ADRT_PRECOND_LORR) return launch_lorr(n, v, out, S, st);
     const int deg = p->d.g1_deg;
-    if (deg - 1 <= 16) return launch_g1_fused(n, deg, 0, v, nullptr, nullptr, 1.0, 0.0, 0.0, out, S, st);
+    if (deg - 1 <= 16)
+        return launch_g1_fused(n, deg, 0, v, nullptr, nullptr, 1.0, 0.0, 0.0, out, S, st, PREC_F64, nullptr,
+                               p->d.max_batch);
     const double lo = 4.0, hi = 16.0;
     const double c = (hi - lo) / (hi + lo), a = 2.0 / (hi + lo);
     // u_j lands in buffer (j % 2) with buffer (deg - 1) % 2 == out
@@ -680,7 +682,7 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
     const double* py = p->y;
     if (fused) {
         ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, p->y, nullptr, nullptr, a, 0.0, 0.0, X[0], S, st,
-                                 (refine > 0 || p->d.g1_fp64) ? PREC_F64 : prec));
+                                 (refine > 0 || p->d.g1_fp64) ? PREC_F64 : prec, nullptr, p->d.max_batch));
     } else {
         if (has_p) {
             ADRT_TRY(precond(p, p->y, p->r, S, st));
@@ -702,7 +704,8 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
                              p->d.l1_fp64 && pk == PREC_F32));
         if (fused) {
             ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, res32 ? nullptr : p->res, xk, xp, a, om, om - 1.0,
-                                     X[k % 2], S, st, pg, res32 ? reinterpret_cast<const float*>(p->res) : nullptr));
+                                     X[k % 2], S, st, pg, res32 ? reinterpret_cast<const float*>(p->res) : nullptr,
+                                     p->d.max_batch));
             continue;
         }
         const double* rr = p->res;
@@ -712,6 +715,117 @@ int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void
         }
         ADRT_TRY(launch_cheb_update(xk, xp, rr, a, om, om - 1.0, X[k % 2], count, st));
     }
+    return OK;
+}
+
+// One outer step of every lane of a continuous batch: adrt_plan_apply's loop turned inside out, so that sinograms
+// can join a batch between steps.  Lanes are ordered oldest first, so the lanes of one kind of step are a contiguous
+// run: FP64 refinement steps, then the plan's own steps, then the sinograms that start (step 0).  Each run is one
+// pass over its lanes; the outer update takes each lane's own omega_k.  A lane's arithmetic is that of
+// adrt_plan_apply step for step, so its result does not depend on which lanes share its steps.
+int adrt_plan_tick(adrt_plan* plan, const int* k_lane, int S, const double* d_b, double* d_xcur, double* d_xnext,
+                   void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_TRY(check_batch(p, S));
+    if (S == 0) return OK;
+    ADRT_REQUIRE(k_lane && d_xcur && d_xnext && d_xcur != d_xnext, "plan_tick: null or aliased buffer");
+    ADRT_REQUIRE(S <= G1_MAX_LANES, "plan_tick: at most 64 lanes");
+    const int D = p->d.outer_deg;
+    for (int i = 0; i < S; ++i) {
+        ADRT_REQUIRE(k_lane[i] >= 0 && k_lane[i] < D, "plan_tick: a lane's step must be in [0, outer_deg)");
+        ADRT_REQUIRE(i == 0 || k_lane[i] <= k_lane[i - 1], "plan_tick: lanes must be ordered oldest first");
+    }
+    ADRT_REQUIRE(k_lane[S - 1] > 0 || d_b, "plan_tick: lanes at step 0 need their sinograms (d_b)");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t NN = (size_t)p->N * p->N;
+    const size_t BB = (size_t)4 * (2 * p->N - 1) * p->N;
+    const double c = (p->d.hi - p->d.lo) / (p->d.hi + p->d.lo), a = 2.0 / (p->d.hi + p->d.lo);
+    const int prec = p->d.precision;
+    const int refine = adrt_plan_refine_steps(plan);
+    const bool has_p = p->d.kind == ADRT_KIND_HYBRID && p->d.precond != ADRT_PRECOND_NONE;
+    const bool fused = has_p && p->d.precond == ADRT_PRECOND_G1 && p->d.g1_deg - 1 <= 16;
+    // omega_k by adrt_plan_apply's recurrence, so every lane's weights are the same doubles
+    std::vector<double> omk((size_t)D, 1.0);
+    double om = 1.0;
+    for (int k = 1; k < D; ++k) {
+        om = (k == 1) ? 1.0 / (1.0 - c * c / 2.0) : 1.0 / (1.0 - c * c * om / 4.0);
+        omk[(size_t)k] = om;
+    }
+    auto step_prec = [&](int k) {
+        if (k == 0) return refine > 0 ? PREC_F64 : prec;
+        return (refine > 0 && k >= D - refine) ? PREC_F64 : prec;
+    };
+    for (int i = 0; i < S;) {
+        const bool start = k_lane[i] == 0;
+        const int pk = step_prec(k_lane[i]);
+        int j = i + 1;
+        while (j < S && (k_lane[j] == 0) == start && step_prec(k_lane[j]) == pk) ++j;
+        const int G = j - i;
+        double* y = p->y + i * NN;
+        double* xn = d_xnext + i * NN;
+        if (start) {
+            ADRT_TRY(packed_pass(p, pk, 1, d_b + i * BB, p->ops_back, p->nback, nullptr, y, G, st, false,
+                                 p->d.l1_fp64 && pk == PREC_F32));
+            if (fused) {
+                ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 1, y, nullptr, nullptr, a, 0.0, 0.0, xn, G, st,
+                                         (refine > 0 || p->d.g1_fp64) ? PREC_F64 : prec, nullptr, p->d.max_batch));
+            } else {
+                const double* py = y;
+                if (has_p) {
+                    ADRT_TRY(precond(p, y, p->r, G, st));
+                    py = p->r;
+                }
+                ADRT_TRY(launch_scale(py, a, xn, (long long)G * NN, st));
+            }
+            i = j;
+            continue;
+        }
+        const double* xk = d_xcur + i * NN;
+        const int pg = p->d.g1_fp64 ? PREC_F64 : pk;
+        const bool res32 = fused && pg == PREC_F32;
+        ADRT_TRY(packed_pass(p, pk, 0, xk, p->ops_op, p->nop, y, p->res, G, st, res32,
+                             p->d.l1_fp64 && pk == PREC_F32));
+        if (fused) {
+            G1Lanes ln;
+            for (int l = 0; l < G; ++l) {
+                const int k = k_lane[i + l];
+                ln.w1[l] = omk[(size_t)k];
+                ln.w0[l] = omk[(size_t)k] - 1.0;
+                if (k >= 2) ln.has_xp |= 1ull << l;
+            }
+            ADRT_TRY(launch_g1_fused(p->d.n, p->d.g1_deg, 2, res32 ? nullptr : p->res, xk, xn, a, 0.0, 0.0, xn, G,
+                                     st, pg, res32 ? reinterpret_cast<const float*>(p->res) : nullptr,
+                                     p->d.max_batch, &ln));
+        } else {
+            const double* rr = p->res;
+            if (has_p) {
+                ADRT_TRY(precond(p, p->res, p->r, G, st));
+                rr = p->r;
+            }
+            for (int l = 0; l < G; ++l) {
+                const int k = k_lane[i + l];
+                ADRT_TRY(launch_cheb_update(xk + l * NN, k >= 2 ? xn + l * NN : nullptr, rr + l * NN, a,
+                                            omk[(size_t)k], omk[(size_t)k] - 1.0, xn + l * NN, (long long)NN, st));
+            }
+        }
+        i = j;
+    }
+    return OK;
+}
+
+// Moves a lane of a continuous batch: its right-hand side (kept in the plan) and its two iterates.
+int adrt_plan_lane_move(adrt_plan* plan, int from, int to, double* d_xa, double* d_xb, void* stream) {
+    auto* p = (Plan*)plan;
+    ADRT_REQUIRE(p, "null plan");
+    ADRT_REQUIRE(d_xa && d_xb, "plan_lane_move: null buffer");
+    ADRT_REQUIRE(from >= 0 && to >= 0 && from < p->d.max_batch && to < p->d.max_batch,
+                 "plan_lane_move: lanes must be in [0, max_batch)");
+    if (from == to) return OK;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t NN = (size_t)p->N * p->N, bytes = NN * sizeof(double);
+    ADRT_HIP(hipMemcpyAsync(p->y + to * NN, p->y + from * NN, bytes, hipMemcpyDeviceToDevice, st));
+    ADRT_HIP(hipMemcpyAsync(d_xa + to * NN, d_xa + from * NN, bytes, hipMemcpyDeviceToDevice, st));
+    ADRT_HIP(hipMemcpyAsync(d_xb + to * NN, d_xb + from * NN, bytes, hipMemcpyDeviceToDevice, st));
     return OK;
 }
 

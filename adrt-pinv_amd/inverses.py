@@ -129,6 +129,17 @@ class _PlanBase:
             for _ in range(self.outer_deg - 1):
                 self._count(self.ops_op(), S)
 
+    def tick_device(self, ks, d_b, d_xcur, d_xnext, stream=None):
+        """One outer step of every lane of a continuous batch (adrt_plan_tick): lane i at step ks[i], oldest first."""
+        S = len(ks)
+        arr = (L.I * max(S, 1))(*[int(k) for k in ks])
+        L.check(L.load().adrt_plan_tick(self._plan, arr, S, L.ptr(d_b) if d_b is not None else None, L.ptr(d_xcur),
+                                        L.ptr(d_xnext), stream.handle if stream else None), "plan_tick")
+
+    def lane_move_device(self, src, dst, d_xa, d_xb, stream=None):
+        L.check(L.load().adrt_plan_lane_move(self._plan, int(src), int(dst), L.ptr(d_xa), L.ptr(d_xb),
+                                             stream.handle if stream else None), "plan_lane_move")
+
     # ---- host (numpy) entry points, one sinogram or image at a time like the reference
     def _img_call(self, fn, x):
         x = np.ascontiguousarray(x, dtype=float).reshape(1, self.N, self.N)

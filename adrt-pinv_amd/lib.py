@@ -80,6 +80,8 @@ SIGNATURES = {
     "adrt_cgls": (I, [I, P, P, I, D, I, I, P, P, P, I, P]),
     "adrt_cgls_cb": (I, [I, P, P, I, D, I, I, P, P, P, I, P, P, P]),
     "adrt_plan_apply": (I, [P, P, P, I, P]),
+    "adrt_plan_tick": (I, [P, ctypes.POINTER(I), I, P, P, P, P]),
+    "adrt_plan_lane_move": (I, [P, I, I, P, P, P]),
     "adrt_graph_begin": (I, [P]),
     "adrt_graph_end": (I, [P, ctypes.POINTER(P)]),
     "adrt_graph_launch": (I, [P, P]),

@@ -136,10 +136,169 @@ class GpuSlot:
             self.streams.close()
 
 
+class StepLanes:
+    """The device side of a StepSlot: a resident plan whose ``max_batch`` lanes hold sinograms at different outer
+    steps, two iterate buffers, the lanes' sinograms, and per queued tick page-locked staging and an event.  The plan
+    is a hybrid plan made from ``setup``, or any plan (hybrid or L_p) given as ``plan``."""
+
+    def __init__(self, n, cap, precision, refine, max_batch, setup, streams, depth, plan=None, **kw):
+        from . import lib as L
+        from .inverses import FastHybridHip
+
+        self.L = L
+        N = 1 << n
+        self.own_plan = plan is None
+        if plan is None:
+            lam, outer = setup
+            plan = FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=cap, precision=precision,
+                                 max_batch=max_batch, refine=refine, **kw)
+        self.plan = plan
+        max_batch = plan.max_batch
+        self.outer_deg = self.plan.outer_deg
+        self.max_batch = max_batch
+        self.own_streams = streams is None
+        self.streams = streams or SizeStreams(own_queue=False)
+        self.d_b = L.DeviceArray((max_batch, 4, 2 * N - 1, N))
+        self.d_x = [L.DeviceArray((max_batch, N, N)), L.DeviceArray((max_batch, N, N))]
+        self.host_b = [L.PinnedArray((max_batch, 4, 2 * N - 1, N)) for _ in range(depth)]
+        self.host_x = [L.PinnedArray((max_batch, N, N)) for _ in range(depth)]
+        self.events = [L.Event() for _ in range(depth)]
+
+    def device_bytes(self):
+        return self.plan.plan_bytes + self.d_b.nbytes + 2 * self.d_x[0].nbytes
+
+    def shared_bytes(self):
+        return getattr(self.plan, "table_bytes", 0)
+
+    def upload(self, u, lane0, sinos):
+        L, st = self.L, self.streams.compute
+        hb = self.host_b[u]
+        for j, b in enumerate(sinos):
+            hb.a[j] = b
+        row = hb.nbytes // self.max_batch
+        L.check(L.load().adrt_memcpy(self.d_b.ptr + lane0 * row, hb.ptr, len(sinos) * row, 1, st.handle), "h2d")
+
+    def tick(self, ks, cur):
+        self.plan.tick_device(ks, self.d_b, self.d_x[cur], self.d_x[1 - cur], self.streams.compute)
+
+    def download(self, u, buf, count):
+        L = self.L
+        row = self.host_x[u].nbytes // self.max_batch
+        L.check(L.load().adrt_memcpy(self.host_x[u].ptr, self.d_x[buf].ptr, count * row, 4,
+                                     self.streams.compute.handle), "d2h")
+
+    def move(self, src, dst):
+        self.plan.lane_move_device(src, dst, self.d_x[0], self.d_x[1], self.streams.compute)
+
+    def record(self, u):
+        self.events[u].record(self.streams.compute)
+
+    def done(self, u):
+        return self.events[u].done()
+
+    def result(self, u, j):
+        return self.host_x[u].a[j].copy()
+
+    def close(self):
+        self.streams.compute.sync()
+        if self.own_plan:
+            self.plan.close()
+        for h in self.host_b + self.host_x:
+            h.free()
+        if self.own_streams:
+            self.streams.close()
+
+
+class StepSlot:
+    """Continuous batching at outer-step granularity for one (N, p): ``max_batch`` lanes hold sinograms at different
+    outer steps, and every tick (adrt_plan_tick) advances each lane by one step.  The lanes at FP64 refinement
+    steps, those at the plan's own steps and those that start share one pass per kind: the row launch, and with it
+    the stream of the dense tables, serves all of them.  A request joins at the next tick once a lane is free,
+    instead of waiting for a whole batch to finish, and leaves after its last step; its result is the same bits as
+    the plan's apply of that sinogram alone.  ``DEPTH`` ticks are kept queued on the size's compute stream, so the
+    device does not wait for the host between ticks.
+
+    A tick, in stream order: the new sinograms upload into the lanes after the running ones; the tick; the lanes
+    that took their last step (the oldest, so a prefix of the lanes) download; the running lanes move down over them
+    (adrt_plan_lane_move); an event.  ``dev`` is the device side (StepLanes; tests use a host stand-in)."""
+
+    continuous = True
+    DEPTH = 2
+
+    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=8, setup=None, streams=None, dev=None, **kw):
+        self.n, self.cap, self.max_batch = n, cap, max_batch
+        self.dev = dev or StepLanes(n, cap, precision, refine, max_batch, setup, streams, self.DEPTH, **kw)
+        max_batch = self.max_batch = self.dev.max_batch
+        self.plan = getattr(self.dev, "plan", None)
+        self.streams = getattr(self.dev, "streams", None)
+        self.D = self.dev.outer_deg
+        self.lanes = []     # [request, step of its next tick], oldest first: device lane i holds lanes[i]
+        self.cur = 0        # iterate buffer holding the lanes' current iterates
+        self.inflight = []  # (staging index, finishing requests) of the queued ticks, oldest first
+        self.ticks = 0
+
+    @property
+    def busy(self):
+        return bool(self.lanes) or bool(self.inflight)
+
+    def free_lanes(self):
+        return self.max_batch - len(self.lanes)
+
+    def device_bytes(self):
+        return self.dev.device_bytes()
+
+    def shared_bytes(self):
+        return self.dev.shared_bytes()
+
+    def can_tick(self):
+        return len(self.inflight) < self.DEPTH
+
+    def tick(self, new_reqs=()):
+        """Queue one tick: the lanes advance one step, ``new_reqs`` (at most free_lanes()) join at step 0."""
+        new_reqs = list(new_reqs)
+        if len(new_reqs) > self.free_lanes():
+            raise ValueError("more new sinograms than free lanes")
+        if not self.can_tick():
+            raise RuntimeError("tick queue full")
+        u = self.ticks % self.DEPTH
+        self.ticks += 1
+        if new_reqs:
+            self.dev.upload(u, len(self.lanes), [r.b for r in new_reqs])
+            self.lanes.extend([r, 0] for r in new_reqs)
+        if not self.lanes:
+            return
+        ks = [k for _, k in self.lanes]
+        self.dev.tick(ks, self.cur)
+        nxt = 1 - self.cur
+        nf = sum(1 for k in ks if k == self.D - 1)
+        if nf:
+            self.dev.download(u, nxt, nf)
+            for i in range(nf, len(ks)):
+                self.dev.move(i, i - nf)
+        self.dev.record(u)
+        finished = [r for r, _ in self.lanes[:nf]]
+        self.lanes = [[r, k + 1] for r, k in self.lanes[nf:]]
+        self.cur = nxt
+        self.inflight.append((u, finished))
+
+    def poll(self):
+        """The requests whose last tick has completed."""
+        done = []
+        while self.inflight and self.dev.done(self.inflight[0][0]):
+            u, finished = self.inflight.pop(0)
+            for j, r in enumerate(finished):
+                r.x = self.dev.result(u, j)
+                done.append(r)
+        return done
+
+    def close(self):
+        self.dev.close()
+
+
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
                  precision="fp32", refine=1, tune=True, stream_per_size=True, max_slots_per_key=None,
-                 max_wait_s=None, separate_upload=False, **kw):
+                 max_wait_s=None, separate_upload=False, continuous=(), **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
         deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory;
         a (N, p)'s tables are counted with its first slot).  slots_per_key slots per key are made at start while they
@@ -148,6 +307,8 @@ class Server:
         stream_per_size: a size's inverses run on a stream with a hardware queue of its own.  separate_upload: a
         size's uploads go on an ordinary stream of their own instead of the compute stream.
         max_wait_s: a request not started within that many seconds is shed with an error (None: requests wait).
+        continuous: the sizes n whose keys run one StepSlot (requests join a running batch between outer steps)
+        instead of batches in slots; max_batch is then its number of lanes.
 
         Measured on one MI355X at 200 req/s over N in {128 .. 1024}, p in {16, 32}: uploads on the compute stream
         and two slots per key give p50 / p99 10 / 120 ms; a separate upload stream 38 / 145 ms (HIP spreads
@@ -166,6 +327,7 @@ class Server:
         self._streams = {}
         sizes = sorted({n for n, _ in self.keys})
         self._mb = (lambda n: max_batch.get(n, 4)) if isinstance(max_batch, dict) else (lambda n: max_batch)
+        self.continuous = set(continuous)
         if stream_per_size and slot_factory is None:
             from . import lib as L
 
@@ -177,7 +339,8 @@ class Server:
             if n not in self._streams:
                 self._streams[n] = SizeStreams(own_queue=stream_per_size, separate_upload=separate_upload)
             rf = refine.get((n, cap), 1) if isinstance(refine, dict) else refine
-            return GpuSlot(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n], **kw)
+            cls = StepSlot if n in self.continuous else GpuSlot
+            return cls(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n], **kw)
 
         self._factory = slot_factory or gpu_slot
         if mem_budget is None and slot_factory is None:
@@ -192,6 +355,8 @@ class Server:
         self._slot_bytes = {}
         for rnd in range(slots_per_key):
             for k in self.keys:
+                if rnd > 0 and k[0] in self.continuous:
+                    continue  # one slot of lanes per continuous key
                 if rnd > 0 and not self._fits(k):
                     continue  # admission by memory: no further instances of this key
                 self._add_slot(k, prepare=False)
@@ -253,6 +418,13 @@ class Server:
             r.t_start = t
         slot.launch(batch)
 
+    def _tick(self, slot, q):
+        new = [q.popleft() for _ in range(min(slot.free_lanes(), len(q)))]
+        t = time.perf_counter()
+        for r in new:
+            r.t_start = t
+        slot.tick(new)
+
     def step(self):
         now = time.perf_counter()
         for s in self.slots:
@@ -262,10 +434,15 @@ class Server:
         self._shed_old(now)
         for s in self.slots:
             q = self.pending[(s.n, s.cap)]
-            if q and not s.busy:
+            if getattr(s, "continuous", False):
+                while s.can_tick() and (s.lanes or q):
+                    self._tick(s, q)
+            elif q and not s.busy:
                 self._start(s, q)
         # requests still waiting with every slot of their key busy: one more slot, where memory holds it
         for k, q in self.pending.items():
+            if k[0] in self.continuous:
+                continue
             if q and sum(1 for s in self.slots if (s.n, s.cap) == k) < self.max_slots and self._fits(k):
                 self.grown += 1
                 self._start(self._add_slot(k), q)

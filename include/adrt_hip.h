@@ -209,6 +209,19 @@ int adrt_plan_time_rows_count(adrt_plan* plan, long long* stamped);
 int adrt_plan_refine_steps(adrt_plan* plan);
 /* x = L b, the whole inverse (fastops.py:287-288 / 337-338); d_x must not alias d_b */
 int adrt_plan_apply(adrt_plan* plan, const double* d_b, double* d_x, int S, void* stream);
+/* Continuous batching at outer-step granularity.  No counterpart in the reference (its inverses take one sinogram
+ * per call); for servers that let sinograms join a running batch between outer steps instead of waiting for the
+ * whole batch.  One outer step of adrt_plan_apply for every lane i < S (S <= min(max_batch, 64)): lane i is at step
+ * k_lane[i] (0 <= k < outer_deg), lanes ordered oldest first (k_lane non-increasing).  A lane at step 0 reads its
+ * sinogram from d_b (lane i at d_b + i * 4 (2N-1) N) and keeps its right-hand side in the plan; a lane at step k >= 1
+ * reads x_(k-1) from d_xcur and, for k >= 2, x_(k-2) from d_xnext.  Every lane writes its x_k to d_xnext (lane i at
+ * offset i N^2), and the caller swaps d_xcur and d_xnext between ticks.  After outer_deg ticks a lane holds in
+ * d_xnext the same bits adrt_plan_apply gives for its sinogram. */
+int adrt_plan_tick(adrt_plan* plan, const int* k_lane, int S, const double* d_b, double* d_xcur, double* d_xnext,
+                   void* stream);
+/* moves lane `from` of a continuous batch to lane `to`: its right-hand side (in the plan) and its images in d_xa and
+ * d_xb (the two iterate buffers of adrt_plan_tick); no counterpart in the reference */
+int adrt_plan_lane_move(adrt_plan* plan, int from, int to, double* d_xa, double* d_xb, void* stream);
 
 /* hipGraph capture of everything enqueued on `stream` (and streams joined through adrt_stream_wait). */
 typedef struct adrt_graph adrt_graph;

@@ -238,6 +238,118 @@ def test_server_sheds_requests_that_wait_too_long():
     assert all(r.error is None and r.x is not None for r in srv.done)
 
 
+class HostLanes:
+    """Host stand-in for StepLanes with adrt_plan_tick's contract on a toy outer iteration: lane i at step k keeps
+    its right-hand side y, reads x_(k-1) from the current buffer and x_(k-2) from the next one, writes x_k to the
+    next one; ticks complete after `latency` polls of their event."""
+
+    def __init__(self, D, max_batch, m=3, latency=2):
+        self.outer_deg, self.max_batch, self.latency = D, max_batch, latency
+        self.b = np.zeros((max_batch, m))
+        self.y = np.full((max_batch, m), np.nan)
+        self.x = [np.full((max_batch, m), np.nan), np.full((max_batch, m), np.nan)]
+        self.staged = [np.zeros((max_batch, m)) for _ in range(2)]
+        self.polls = {}
+        self.lane_counts = []
+
+    @staticmethod
+    def step(k, y, xk, xp):
+        return 2.0 * y if k == 0 else 0.5 * xk + (0.25 * xp if k >= 2 else 0.0) + y + k
+
+    @classmethod
+    def apply(cls, b, D):
+        xs = [cls.step(0, b, None, None)]
+        for k in range(1, D):
+            xs.append(cls.step(k, b, xs[-1], xs[-2] if k >= 2 else None))
+        return xs[-1]
+
+    def device_bytes(self):
+        return 1000
+
+    def shared_bytes(self):
+        return 0
+
+    def upload(self, u, lane0, sinos):
+        for j, b in enumerate(sinos):
+            self.b[lane0 + j] = b
+
+    def tick(self, ks, cur):
+        assert all(a >= b for a, b in zip(ks, ks[1:])) and 0 <= min(ks) and max(ks) < self.outer_deg
+        self.lane_counts.append(len(ks))
+        xc, xn = self.x[cur], self.x[1 - cur]
+        for i, k in enumerate(ks):
+            if k == 0:
+                self.y[i] = self.b[i]
+            xn[i] = self.step(k, self.y[i], xc[i], xn[i])
+
+    def download(self, u, buf, count):
+        self.staged[u][:count] = self.x[buf][:count]
+
+    def move(self, src, dst):
+        for a in (self.y, self.x[0], self.x[1]):
+            a[dst] = a[src]
+            a[src] = np.nan
+
+    def record(self, u):
+        self.polls[u] = 0
+
+    def done(self, u):
+        self.polls[u] += 1
+        return self.polls[u] >= self.latency
+
+    def result(self, u, j):
+        return self.staged[u][j].copy()
+
+    def close(self):
+        pass
+
+
+def test_step_slot_lanes_join_and_leave_between_steps():
+    """StepSlot (continuous batching at outer-step granularity) on the host stand-in: requests that arrive while
+    others are mid-iteration join at the next tick, lanes stay ordered oldest first, finished lanes leave from the
+    front while the others move down, and every result equals the iteration run alone."""
+    S = pkg("serve")
+    D, lanes = 7, 4
+    made = []
+
+    def factory(n, cap, mb):
+        s = S.StepSlot(n, cap, max_batch=mb, dev=HostLanes(D, mb))
+        made.append(s)
+        return s
+
+    srv = S.Server([(10, 16)], slots_per_key=2, max_batch=lanes, mem_budget=10**9, slot_factory=factory,
+                   continuous={10})
+    assert len(srv.slots) == 1  # one slot of lanes per continuous key
+    rng = np.random.default_rng(3)
+    reqs = [S.Request(i, 10, 16, rng.standard_normal(3), 0.0) for i in range(11)]
+    arrive = {0: [0, 1], 3: [2], 4: [3, 4, 5], 9: [6, 7, 8, 9, 10]}
+    for t in range(200):
+        for i in arrive.get(t, []):
+            srv.submit(reqs[i])
+        srv.step()
+        if t > max(arrive) and srv.idle():
+            break
+    assert srv.idle() and len(srv.done) == len(reqs) and not srv.shed
+    for r in reqs:
+        np.testing.assert_array_equal(r.x, HostLanes.apply(r.b, D))
+    dev = made[0].dev
+    assert max(dev.lane_counts) == lanes and min(dev.lane_counts) >= 1
+    # requests 2..5 arrived while 0 and 1 were mid-iteration and shared their ticks
+    assert reqs[2].t_start < reqs[0].t_done and reqs[5].t_start < reqs[1].t_done
+
+
+def test_step_slot_rejects_overfull_ticks():
+    S = pkg("serve")
+    s = S.StepSlot(10, 16, max_batch=2, dev=HostLanes(5, 2))
+    rs = [S.Request(i, 10, 16, np.ones(3), 0.0) for i in range(3)]
+    with pytest.raises(ValueError):
+        s.tick(rs)
+    s.tick(rs[:2])
+    s.tick()
+    with pytest.raises(RuntimeError):  # DEPTH ticks already queued
+        s.tick()
+
+
 def test_poisson_trace_and_stats():
     S = pkg("serve")
     pool = {7: [np.zeros(2)], 8: [np.ones(2)]}
