@@ -15,9 +15,13 @@ server's answer to overload); started requests always complete.  The loop
   * grows the slots of keys whose requests still wait, as memory allows, and launches on them at once,
 
 so requests that arrive while others are on the GPU start as soon as a slot of their size frees up or fits in memory,
-and the batches of different sizes overlap on the device (a size's compute stream has a hardware queue of its own, adrt_stream_create_own_queue, so a small-N batch
-never waits behind an N=1024 batch's kernels on one queue).  Requests for a size the server does not hold are
-rejected with an error; the others are unaffected.
+and the batches of different sizes overlap on the device (a size's compute stream has a hardware queue of its own,
+adrt_stream_create_own_queue, so a small-N batch never waits behind an N=1024 batch's kernels on one queue).
+Requests for a size the server does not hold are rejected with an error; the others are unaffected.
+
+Sizes named in ``continuous`` are served at outer-step granularity instead: one StepSlot per (N, p) whose lanes hold
+sinograms at different outer steps (adrt_plan_tick), so a request joins a running batch at the next outer step rather
+than waiting for the batch to finish.
 
 The slot factory is pluggable so the scheduling logic can be exercised without a GPU (tests use a host stand-in).
 """
@@ -29,10 +33,15 @@ import numpy as np
 
 
 class Request:
-    __slots__ = ("rid", "n", "cap", "b", "t_arrive", "t_start", "t_done", "x", "error")
+    """One inversion: sinogram ``b`` of size N = 2^n with dense cap ``cap``.  ``b_ptr``: the host address of ``b``
+    when it already sits in page-locked memory (lib.PinnedArray); its upload then reads it there, with no staging
+    copy, and the caller keeps it unchanged until the request is done."""
 
-    def __init__(self, rid, n, cap, b, t_arrive):
+    __slots__ = ("rid", "n", "cap", "b", "b_ptr", "t_arrive", "t_start", "t_done", "x", "error")
+
+    def __init__(self, rid, n, cap, b, t_arrive, b_ptr=None):
         self.rid, self.n, self.cap, self.b, self.t_arrive = rid, n, cap, b, t_arrive
+        self.b_ptr = b_ptr
         self.t_start = self.t_done = None
         self.x = None
         self.error = None
@@ -105,12 +114,18 @@ class GpuSlot:
         S = len(reqs)
         L, lib = self.L, self.L.load()
         ss = self.streams
+        row = self.host_b.nbytes // self.max_batch
+        staged = []
         for j, r in enumerate(reqs):
-            self.host_b.a[j] = r.b
+            if r.b_ptr is not None:  # page-locked already: uploaded from where it is
+                L.check(lib.adrt_memcpy(self.d_b.ptr + j * row, r.b_ptr, row, 1, ss.up.handle), "h2d")
+            else:
+                self.host_b.a[j] = r.b
+                staged.append(j)
         g = self.graph(S)
-        sb = S * (self.host_b.nbytes // self.max_batch)
         sx = S * (self.host_x.nbytes // self.max_batch)
-        L.check(lib.adrt_memcpy(self.d_b.ptr, self.host_b.ptr, sb, 1, ss.up.handle), "h2d")
+        for j in staged:
+            L.check(lib.adrt_memcpy(self.d_b.ptr + j * row, self.host_b.ptr + j * row, row, 1, ss.up.handle), "h2d")
         if ss.up is not ss.compute:
             self.ev_up.record(ss.up)
             ss.compute.wait_event(self.ev_up)
@@ -138,8 +153,10 @@ class GpuSlot:
 
 class StepLanes:
     """The device side of a StepSlot: a resident plan whose ``max_batch`` lanes hold sinograms at different outer
-    steps, two iterate buffers, the lanes' sinograms, and per queued tick page-locked staging and an event.  The plan
-    is a hybrid plan made from ``setup``, or any plan (hybrid or L_p) given as ``plan``."""
+    steps, two iterate buffers and the lanes' sinograms on the device; page-locked rings of ``max_batch`` sinograms
+    and images for the uploads and downloads; an event per queued tick.  A ring entry is reused once the tick that
+    used it has completed (the slot waits for that tick in the rare case it has not).  The plan is a hybrid plan made
+    from ``setup``, or any plan (hybrid or L_p) given as ``plan``."""
 
     def __init__(self, n, cap, precision, refine, max_batch, setup, streams, depth, plan=None, **kw):
         from . import lib as L
@@ -156,13 +173,24 @@ class StepLanes:
         max_batch = plan.max_batch
         self.outer_deg = self.plan.outer_deg
         self.max_batch = max_batch
+        self.depth = depth
         self.own_streams = streams is None
         self.streams = streams or SizeStreams(own_queue=False)
         self.d_b = L.DeviceArray((max_batch, 4, 2 * N - 1, N))
         self.d_x = [L.DeviceArray((max_batch, N, N)), L.DeviceArray((max_batch, N, N))]
-        self.host_b = [L.PinnedArray((max_batch, 4, 2 * N - 1, N)) for _ in range(depth)]
-        self.host_x = [L.PinnedArray((max_batch, N, N)) for _ in range(depth)]
+        # at most 2 max_batch sinograms join within the queued ticks (the lanes, and as many again after lanes
+        # finish) and at most max_batch images await reading (a lane cannot start and finish within them)
+        self.nb = 2 * max_batch
+        self.host_b = L.PinnedArray((self.nb, 4, 2 * N - 1, N))
+        self.host_x = L.PinnedArray((max_batch, N, N))
+        self.b_row = self.host_b.nbytes // self.nb
+        self.x_row = self.host_x.nbytes // max_batch
+        self.b_owner = [-1] * self.nb  # tick that last uploaded from each ring entry
+        self.x_unread = [False] * max_batch  # image entries downloaded into and not yet read
+        self.b_next = self.x_next = 0
         self.events = [L.Event() for _ in range(depth)]
+        self.recorded = {}  # tick -> event index, for the queued ticks
+        self.completed = -1
 
     def device_bytes(self):
         return self.plan.plan_bytes + self.d_b.nbytes + 2 * self.d_x[0].nbytes
@@ -170,41 +198,67 @@ class StepLanes:
     def shared_bytes(self):
         return getattr(self.plan, "table_bytes", 0)
 
-    def upload(self, u, lane0, sinos):
+    def _free(self, owner):
+        while owner > self.completed and not self.done(owner):
+            pass
+
+    def upload(self, tick, lane, b, b_ptr=None):
         L, st = self.L, self.streams.compute
-        hb = self.host_b[u]
-        for j, b in enumerate(sinos):
-            hb.a[j] = b
-        row = hb.nbytes // self.max_batch
-        L.check(L.load().adrt_memcpy(self.d_b.ptr + lane0 * row, hb.ptr, len(sinos) * row, 1, st.handle), "h2d")
+        if b_ptr is not None:  # page-locked already: uploaded from where it is
+            L.check(L.load().adrt_memcpy(self.d_b.ptr + lane * self.b_row, b_ptr, self.b_row, 1, st.handle), "h2d")
+            return
+        e = self.b_next
+        self.b_next = (e + 1) % self.nb
+        self._free(self.b_owner[e])
+        self.host_b.a[e] = b
+        self.b_owner[e] = tick
+        L.check(L.load().adrt_memcpy(self.d_b.ptr + lane * self.b_row, self.host_b.ptr + e * self.b_row, self.b_row,
+                                     1, st.handle), "h2d")
 
     def tick(self, ks, cur):
         self.plan.tick_device(ks, self.d_b, self.d_x[cur], self.d_x[1 - cur], self.streams.compute)
 
-    def download(self, u, buf, count):
+    def download(self, tick, buf, lane):
+        """Queues the download of a lane's image; returns the ring entry that will hold it."""
         L = self.L
-        row = self.host_x[u].nbytes // self.max_batch
-        L.check(L.load().adrt_memcpy(self.host_x[u].ptr, self.d_x[buf].ptr, count * row, 4,
-                                     self.streams.compute.handle), "d2h")
+        e = self.x_next
+        self.x_next = (e + 1) % self.max_batch
+        if self.x_unread[e]:  # cannot happen while outer_deg > depth: at most max_batch lanes finish per window
+            raise RuntimeError("image ring full: a finished lane's image has not been read")
+        self.x_unread[e] = True
+        L.check(L.load().adrt_memcpy(self.host_x.ptr + e * self.x_row, self.d_x[buf].ptr + lane * self.x_row,
+                                     self.x_row, 4, self.streams.compute.handle), "d2h")
+        return e
 
     def move(self, src, dst):
         self.plan.lane_move_device(src, dst, self.d_x[0], self.d_x[1], self.streams.compute)
 
-    def record(self, u):
-        self.events[u].record(self.streams.compute)
+    def record(self, tick):
+        i = tick % self.depth
+        self.events[i].record(self.streams.compute)
+        self.recorded[tick] = i
 
-    def done(self, u):
-        return self.events[u].done()
+    def done(self, tick):
+        if tick <= self.completed:
+            return True
+        if not self.events[self.recorded[tick]].done():
+            return False
+        for t in [t for t in self.recorded if t <= tick]:
+            del self.recorded[t]
+        self.completed = tick
+        return True
 
-    def result(self, u, j):
-        return self.host_x[u].a[j].copy()
+    def result(self, e):
+        x = self.host_x.a[e].copy()
+        self.x_unread[e] = False
+        return x
 
     def close(self):
         self.streams.compute.sync()
         if self.own_plan:
             self.plan.close()
-        for h in self.host_b + self.host_x:
-            h.free()
+        self.host_b.free()
+        self.host_x.free()
         if self.own_streams:
             self.streams.close()
 
@@ -215,26 +269,30 @@ class StepSlot:
     steps, those at the plan's own steps and those that start share one pass per kind: the row launch, and with it
     the stream of the dense tables, serves all of them.  A request joins at the next tick once a lane is free,
     instead of waiting for a whole batch to finish, and leaves after its last step; its result is the same bits as
-    the plan's apply of that sinogram alone.  ``DEPTH`` ticks are kept queued on the size's compute stream, so the
-    device does not wait for the host between ticks.
+    the plan's apply of that sinogram alone.  Up to ``depth`` ticks are kept queued on the size's compute stream
+    (default 16, at most outer_deg - 1: about 10 ms of device work at N = 1024), so the device does not wait for the
+    host between ticks, nor while the host loop stages other slots' sinograms; a new request joins behind them.
 
     A tick, in stream order: the new sinograms upload into the lanes after the running ones; the tick; the lanes
     that took their last step (the oldest, so a prefix of the lanes) download; the running lanes move down over them
     (adrt_plan_lane_move); an event.  ``dev`` is the device side (StepLanes; tests use a host stand-in)."""
 
     continuous = True
-    DEPTH = 2
 
-    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=8, setup=None, streams=None, dev=None, **kw):
-        self.n, self.cap, self.max_batch = n, cap, max_batch
-        self.dev = dev or StepLanes(n, cap, precision, refine, max_batch, setup, streams, self.DEPTH, **kw)
-        max_batch = self.max_batch = self.dev.max_batch
+    def __init__(self, n, cap, precision="fp32", refine=1, max_batch=8, setup=None, streams=None, dev=None,
+                 depth=None, **kw):
+        self.n, self.cap = n, cap
+        depth = depth or 16
+        self.dev = dev or StepLanes(n, cap, precision, refine, max_batch, setup, streams, depth, **kw)
+        self.max_batch = self.dev.max_batch
         self.plan = getattr(self.dev, "plan", None)
         self.streams = getattr(self.dev, "streams", None)
         self.D = self.dev.outer_deg
+        # a lane cannot start and finish within the queued ticks, so at most max_batch images await reading
+        self.depth = max(1, min(depth, self.dev.depth, self.D - 1))
         self.lanes = []     # [request, step of its next tick], oldest first: device lane i holds lanes[i]
         self.cur = 0        # iterate buffer holding the lanes' current iterates
-        self.inflight = []  # (staging index, finishing requests) of the queued ticks, oldest first
+        self.inflight = []  # (tick, [(request, ring entry of its image)]) of the queued ticks, oldest first
         self.ticks = 0
 
     @property
@@ -251,7 +309,7 @@ class StepSlot:
         return self.dev.shared_bytes()
 
     def can_tick(self):
-        return len(self.inflight) < self.DEPTH
+        return len(self.inflight) < self.depth
 
     def tick(self, new_reqs=()):
         """Queue one tick: the lanes advance one step, ``new_reqs`` (at most free_lanes()) join at step 0."""
@@ -260,34 +318,33 @@ class StepSlot:
             raise ValueError("more new sinograms than free lanes")
         if not self.can_tick():
             raise RuntimeError("tick queue full")
-        u = self.ticks % self.DEPTH
-        self.ticks += 1
-        if new_reqs:
-            self.dev.upload(u, len(self.lanes), [r.b for r in new_reqs])
-            self.lanes.extend([r, 0] for r in new_reqs)
-        if not self.lanes:
+        if not self.lanes and not new_reqs:
             return
+        t = self.ticks
+        self.ticks += 1
+        for r in new_reqs:
+            self.dev.upload(t, len(self.lanes), r.b, r.b_ptr)
+            self.lanes.append([r, 0])
         ks = [k for _, k in self.lanes]
         self.dev.tick(ks, self.cur)
         nxt = 1 - self.cur
         nf = sum(1 for k in ks if k == self.D - 1)
+        finished = [(self.lanes[i][0], self.dev.download(t, nxt, i)) for i in range(nf)]
         if nf:
-            self.dev.download(u, nxt, nf)
             for i in range(nf, len(ks)):
                 self.dev.move(i, i - nf)
-        self.dev.record(u)
-        finished = [r for r, _ in self.lanes[:nf]]
+        self.dev.record(t)
         self.lanes = [[r, k + 1] for r, k in self.lanes[nf:]]
         self.cur = nxt
-        self.inflight.append((u, finished))
+        self.inflight.append((t, finished))
 
     def poll(self):
         """The requests whose last tick has completed."""
         done = []
         while self.inflight and self.dev.done(self.inflight[0][0]):
-            u, finished = self.inflight.pop(0)
-            for j, r in enumerate(finished):
-                r.x = self.dev.result(u, j)
+            _, finished = self.inflight.pop(0)
+            for r, e in finished:
+                r.x = self.dev.result(e)
                 done.append(r)
         return done
 
@@ -298,7 +355,7 @@ class StepSlot:
 class Server:
     def __init__(self, keys, slots_per_key=2, max_batch=4, mem_budget=None, slot_factory=None, setups=None,
                  precision="fp32", refine=1, tune=True, stream_per_size=True, max_slots_per_key=None,
-                 max_wait_s=None, separate_upload=False, continuous=(), **kw):
+                 max_wait_s=None, separate_upload=False, continuous=(), tick_depth=None, **kw):
         """keys: iterable of (n, cap).  max_batch: sinograms per launch, one number or {n: number} (small N batch
         deeper at little latency cost).  mem_budget: device bytes the slots may hold (default: 60% of free memory;
         a (N, p)'s tables are counted with its first slot).  slots_per_key slots per key are made at start while they
@@ -308,7 +365,8 @@ class Server:
         size's uploads go on an ordinary stream of their own instead of the compute stream.
         max_wait_s: a request not started within that many seconds is shed with an error (None: requests wait).
         continuous: the sizes n whose keys run one StepSlot (requests join a running batch between outer steps)
-        instead of batches in slots; max_batch is then its number of lanes.
+        instead of batches in slots; max_batch is then its number of lanes, tick_depth its queued ticks (None: the
+        StepSlot default).
 
         Measured on one MI355X at 200 req/s over N in {128 .. 1024}, p in {16, 32}: uploads on the compute stream
         and two slots per key give p50 / p99 10 / 120 ms; a separate upload stream 38 / 145 ms (HIP spreads
@@ -324,6 +382,8 @@ class Server:
         self.shed = []
         self.done = []
         self.grown = 0
+        # host seconds of the loop: polling, launching batches, queueing ticks (and the ticks queued)
+        self.host_s = {"poll": 0.0, "launch": 0.0, "tick": 0.0, "ticks": 0}
         self._streams = {}
         sizes = sorted({n for n, _ in self.keys})
         self._mb = (lambda n: max_batch.get(n, 4)) if isinstance(max_batch, dict) else (lambda n: max_batch)
@@ -339,8 +399,10 @@ class Server:
             if n not in self._streams:
                 self._streams[n] = SizeStreams(own_queue=stream_per_size, separate_upload=separate_upload)
             rf = refine.get((n, cap), 1) if isinstance(refine, dict) else refine
-            cls = StepSlot if n in self.continuous else GpuSlot
-            return cls(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n], **kw)
+            if n in self.continuous:
+                return StepSlot(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n],
+                                depth=tick_depth, **kw)
+            return GpuSlot(n, cap, precision, rf, mb, setups[(n, cap)], streams=self._streams[n], **kw)
 
         self._factory = slot_factory or gpu_slot
         if mem_budget is None and slot_factory is None:
@@ -432,13 +494,22 @@ class Server:
                 r.t_done = now
                 self.done.append(r)
         self._shed_old(now)
+        t1 = time.perf_counter()
+        ticks = 0
         for s in self.slots:
             q = self.pending[(s.n, s.cap)]
             if getattr(s, "continuous", False):
+                t = time.perf_counter()
                 while s.can_tick() and (s.lanes or q):
                     self._tick(s, q)
+                    ticks += 1
+                self.host_s["tick"] += time.perf_counter() - t
             elif q and not s.busy:
+                t = time.perf_counter()
                 self._start(s, q)
+                self.host_s["launch"] += time.perf_counter() - t
+        self.host_s["poll"] += t1 - now
+        self.host_s["ticks"] += ticks
         # requests still waiting with every slot of their key busy: one more slot, where memory holds it
         for k, q in self.pending.items():
             if k[0] in self.continuous:
@@ -479,7 +550,8 @@ class Server:
 
 def poisson_trace(rate, duration, sizes, weights, caps, pool, seed=7):
     """Arrivals with exponential gaps (default_rng(seed)); N from ``sizes`` with ``weights``, p uniform over ``caps``;
-    the sinogram of a request is drawn from pool[n] in turn."""
+    the sinogram of a request is drawn from pool[n] in turn (an array, or a lib.PinnedArray: the request then
+    carries its page-locked address)."""
     rng = np.random.default_rng(seed)
     t, out, rid = 0.0, [], 0
     used = {n: 0 for n in sizes}
@@ -491,7 +563,10 @@ def poisson_trace(rate, duration, sizes, weights, caps, pool, seed=7):
         cap = int(caps[rng.integers(len(caps))])
         b = pool[n][used[n] % len(pool[n])]
         used[n] += 1
-        out.append((t, Request(rid, n, cap, b, None)))
+        if hasattr(b, "ptr"):
+            out.append((t, Request(rid, n, cap, b.a, None, b_ptr=b.ptr)))
+        else:
+            out.append((t, Request(rid, n, cap, b, None)))
         rid += 1
     return out
 

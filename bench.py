@@ -10,7 +10,8 @@ are resident in HBM, replayed from one hipGraph.  ``value`` is inverses/s (B * K
 reports the plain FP64 GPU path (one sinogram per call, no graph, no batching), the relative error of the FP32 result
 against the FP64 one, the roofline of the dominant kernel (the register row kernel of the G_h pass, timed with HIP
 the kernel's own dispatch timestamps inside whole applies), the reference C++ row engine timed on the host cores, a 512-sinogram
-uneven batch in one graph launch and a Poisson serving trace over mixed N with p50 / p99 latency.
+uneven batch in one graph launch and a Poisson serving trace over mixed N with p50 / p99 latency, served twice: with
+batches in slots, and with N = 512 and 1024 served by continuous batching at outer-step granularity.
 """
 
 import argparse
@@ -220,46 +221,64 @@ def fp64_relerr(ns, sinos, xs, dense_cap, caps=None):
 
 
 SERVE_BATCH = {7: 16, 8: 16, 9: 8, 10: 8}  # sinograms per launch by log2 N
+SERVE_CONTINUOUS = "9,10"  # log2 N of the sizes of the second serving leg, served at outer-step granularity
+SERVE_LANES = "9:8,10:8"
 
 
 def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.1), caps=(16, 32), slots_per_key=2,
-                max_wait_s=0.25):
+                max_wait_s=0.25, continuous=(), lanes=None, tick_depth=None):
     """Poisson arrivals (default_rng(7)) of FP32 hybrid requests over mixed N and p at each rate; p50 / p99 latency
     and throughput of the served requests (upload and download inside the latency), how many were shed (not admitted
     into device memory within max_wait_s: the server at overload), and the relative error of every served result
-    against the FP64 inverse of its sinogram with the same setup."""
+    against the FP64 inverse of its sinogram with the same setup.  The sizes in ``continuous`` run one StepSlot per
+    key (requests join a running batch between outer steps) with ``lanes[n]`` lanes instead of batches in slots."""
     S = pkg("serve")
     C = pkg("plancache")
     keys = [(n, c) for n in sizes for c in caps]
+    mb = dict(SERVE_BATCH)
+    mb.update({n: v for n, v in (lanes or {}).items() if n in continuous})
     t0 = time.time()
     setups = {k: C.hybrid_setup(k[0], dense_cap=k[1]) for k in keys}
     # FP64 refinement steps per (N, p): the fewest that keep the FP32 result within 1e-6 of FP64 on probe sinograms
     refine = {k: pkg("autotune").refine_for(k[0], k[1], *setups[k]) for k in keys}
-    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=SERVE_BATCH, setups=setups, max_wait_s=max_wait_s,
-                   refine=refine)
+    srv = S.Server(keys, slots_per_key=slots_per_key, max_batch=mb, setups=setups, max_wait_s=max_wait_s,
+                   refine=refine, continuous=set(continuous), tick_depth=tick_depth)
     setup_s = time.time() - t0
-    pool = {n: list(make_sinos(n, 4, 3000, 4000)) for n in sizes}
+    # the requests' sinograms arrive in page-locked host memory, as from a network or IPC receive buffer, so an
+    # upload reads them where they are
+    L = pkg("lib")
+    pool = {}
+    for n in sizes:
+        pool[n] = []
+        for b in make_sinos(n, 4, 3000, 4000):
+            pb = L.PinnedArray(b.shape)
+            pb.a[...] = b
+            pool[n].append(pb)
     # FP64 results of every (sinogram, cap) the trace can draw: the served results are checked against them
-    ns_all, caps_all, b_all = zip(*[(n, c, b) for n in sizes for c in caps for b in pool[n]])
     ref = {}
     I = pkg("inverses")
     for (n, c), (lam, outer) in setups.items():
         H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=c, max_batch=len(pool[n]))
-        for j, x in enumerate(H.apply_batch(np.stack(pool[n]))):
+        for j, x in enumerate(H.apply_batch(np.stack([pb.a for pb in pool[n]]))):
             ref[(n, c, j)] = x
         H.close()
     # warm every slot once (first launches load code objects)
-    warm = [(0.0, S.Request(-1 - i, s.n, s.cap, pool[s.n][0], 0.0)) for i, s in enumerate(srv.slots)]
+    warm = [(0.0, S.Request(-1 - i, s.n, s.cap, pool[s.n][0].a, 0.0, b_ptr=pool[s.n][0].ptr))
+            for i, s in enumerate(srv.slots)]
     srv.run_trace(warm)
     out = []
     for rate in rates:
         srv.done.clear()
         srv.shed.clear()
+        srv.host_s = dict.fromkeys(srv.host_s, 0)
         trace = S.poisson_trace(rate, duration, list(sizes), list(weights), list(caps), pool)
+        t_run = time.perf_counter()
         done = srv.run_trace(trace)
+        t_run = time.perf_counter() - t_run
+        host = {k: (v / t_run if k != "ticks" else v) for k, v in srv.host_s.items()}
         rel, relk = [], {}
         for r in done:
-            j = next(i for i, b in enumerate(pool[r.n]) if b is r.b)
+            j = next(i for i, pb in enumerate(pool[r.n]) if pb.a is r.b)
             x64 = ref[(r.n, r.cap, j)]
             rel.append(float(np.linalg.norm(r.x - x64) / np.linalg.norm(x64)))
             key = f"{1 << r.n},{r.cap}"
@@ -268,18 +287,25 @@ def serving_leg(rates, duration, sizes=(7, 8, 9, 10), weights=(0.4, 0.3, 0.2, 0.
         out.append({"rate_per_s": rate, "duration_s": duration, "offered": len(trace), **st, "shed": len(srv.shed),
                     "relerr_fp32_vs_fp64_max": max(rel) if rel else None,
                     "relerr_fp32_vs_fp64_median": float(np.median(rel)) if rel else None,
-                    "relerr_fp32_vs_fp64_max_by_key": relk})
+                    "relerr_fp32_vs_fp64_max_by_key": relk, "host_loop_fraction": host})
         log(f"serving at {rate:.0f}/s: p50 {st.get('p50_ms', 0):.1f} ms p99 {st.get('p99_ms', 0):.1f} ms, served "
             f"{st.get('throughput_per_s', 0):.0f}/s, shed {len(srv.shed)} of {len(trace)}, relerr max "
-            f"{max(rel) if rel else float('nan'):.2e} ({', '.join(f'{k}: {v:.1e}' for k, v in sorted(relk.items()))})")
+            f"{max(rel) if rel else float('nan'):.2e} ({', '.join(f'{k}: {v:.1e}' for k, v in sorted(relk.items()))})"
+            f"; host loop: poll {host['poll']:.0%} launch {host['launch']:.0%} tick {host['tick']:.0%} "
+            f"({host['ticks']} ticks)")
         for nn, e in st.get("by_size", {}).items():
             log(f"   N={nn}: {e['count']} served, p50 {e['p50_ms']:.1f} p99 {e['p99_ms']:.1f} ms; mean wait "
                 f"{e.get('wait_ms', 0):.1f} ms, in slot {e.get('slot_ms', 0):.1f} ms")
     res = {"mix": "N in {128,256,512,1024} w .4/.3/.2/.1, p in {16,32}", "slots": len(srv.slots),
-           "max_batch": {str(1 << n): b for n, b in SERVE_BATCH.items()},
+           "max_batch": {str(1 << n): b for n, b in mb.items()},
+           "continuous": [1 << n for n in sorted(continuous)],
+           "tick_depth": {str(1 << s.n): s.depth for s in srv.slots if getattr(s, "continuous", False)},
            "max_wait_s": max_wait_s, "fp64_refine_steps": {f"{1 << n},{c}": r for (n, c), r in refine.items()},
            "device_GiB": srv.device_bytes / 2**30, "setup_s": setup_s, "rates": out}
     srv.close()
+    for pbs in pool.values():
+        for pb in pbs:
+            pb.free()
     return res
 
 
@@ -390,6 +416,14 @@ def main():
                     help="seconds of Poisson arrivals per rate (4 s at 200 req/s: about 800 requests behind the p99)")
     ap.add_argument("--serve-max-wait", type=float, default=0.25,
                     help="a request not admitted into device memory within this many seconds is shed")
+    ap.add_argument("--serve-continuous", default=SERVE_CONTINUOUS,
+                    help="a second serving leg, with these sizes (log2 N, comma separated) served by continuous "
+                         "batching at outer-step granularity (requests join a running batch between outer steps); "
+                         "empty: no second leg")
+    ap.add_argument("--serve-lanes", default=SERVE_LANES,
+                    help="lanes of the continuous sizes as n:lanes, comma separated")
+    ap.add_argument("--serve-tick-depth", type=int, default=None,
+                    help="ticks kept queued per continuous slot (default 16)")
     ap.add_argument("--rows-only", action="store_true")
     ap.add_argument("--mapping", default=None,
                     help="with --rows-only: 'loge,dense' (row mapping and dense placement) instead of autotuning")
@@ -510,10 +544,16 @@ def main():
     if not a.no_mixed:
         mixed = mixed_batch_leg()
         log(f"mixed batch: {mixed['inverses_per_s']:.0f} inverses/s")
-    serving = None
+    serving = serving_cont = None
     if not a.no_serve:
-        serving = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_duration,
-                              max_wait_s=a.serve_max_wait)
+        rates = [float(r) for r in a.serve_rates.split(",")]
+        serving = serving_leg(rates, a.serve_duration, max_wait_s=a.serve_max_wait)
+        cont = [int(v) for v in a.serve_continuous.split(",") if v.strip()]
+        if cont:
+            lanes = {int(k): int(v) for k, v in (e.split(":") for e in a.serve_lanes.split(",") if e.strip())}
+            log(f"serving again with N in {[1 << n for n in cont]} at outer-step granularity")
+            serving_cont = serving_leg(rates, a.serve_duration, max_wait_s=a.serve_max_wait, continuous=cont,
+                                       lanes=lanes, tick_depth=a.serve_tick_depth)
 
     large = None if a.no_large else large_leg()
 
@@ -568,6 +608,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_batch": mixed,
         "serving": serving,
+        "serving_continuous": serving_cont,
         "large": large,
     }
     if cpu:

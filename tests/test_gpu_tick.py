@@ -29,8 +29,7 @@ def run_staggered(plan, B, starts):
     """Sinograms B[i] joining at tick starts[i] (as lanes free up) through a StepSlot on ``plan``; returns the
     results in order, each checked by the slot's own bookkeeping."""
     S = pkg("serve")
-    slot = S.StepSlot(plan.n, 0, dev=S.StepLanes(plan.n, 0, None, None, None, None, None, S.StepSlot.DEPTH,
-                                                 plan=plan))
+    slot = S.StepSlot(plan.n, 0, dev=S.StepLanes(plan.n, 0, None, None, None, None, None, 4, plan=plan))
     reqs = [S.Request(i, plan.n, 0, B[i], 0.0) for i in range(len(B))]
     waiting = list(range(len(B)))
     t = 0

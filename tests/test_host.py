@@ -243,14 +243,15 @@ class HostLanes:
     its right-hand side y, reads x_(k-1) from the current buffer and x_(k-2) from the next one, writes x_k to the
     next one; ticks complete after `latency` polls of their event."""
 
-    def __init__(self, D, max_batch, m=3, latency=2):
-        self.outer_deg, self.max_batch, self.latency = D, max_batch, latency
+    def __init__(self, D, max_batch, m=3, latency=2, depth=8):
+        self.outer_deg, self.max_batch, self.latency, self.depth = D, max_batch, latency, depth
         self.b = np.zeros((max_batch, m))
         self.y = np.full((max_batch, m), np.nan)
         self.x = [np.full((max_batch, m), np.nan), np.full((max_batch, m), np.nan)]
-        self.staged = [np.zeros((max_batch, m)) for _ in range(2)]
+        self.staged = {}
         self.polls = {}
         self.lane_counts = []
+        self.queued = []
 
     @staticmethod
     def step(k, y, xk, xp):
@@ -269,9 +270,8 @@ class HostLanes:
     def shared_bytes(self):
         return 0
 
-    def upload(self, u, lane0, sinos):
-        for j, b in enumerate(sinos):
-            self.b[lane0 + j] = b
+    def upload(self, tick, lane, b, b_ptr=None):
+        self.b[lane] = b
 
     def tick(self, ks, cur):
         assert all(a >= b for a, b in zip(ks, ks[1:])) and 0 <= min(ks) and max(ks) < self.outer_deg
@@ -282,23 +282,27 @@ class HostLanes:
                 self.y[i] = self.b[i]
             xn[i] = self.step(k, self.y[i], xc[i], xn[i])
 
-    def download(self, u, buf, count):
-        self.staged[u][:count] = self.x[buf][:count]
+    def download(self, tick, buf, lane):
+        h = (tick, lane)
+        self.staged[h] = self.x[buf][lane].copy()
+        return h
 
     def move(self, src, dst):
         for a in (self.y, self.x[0], self.x[1]):
             a[dst] = a[src]
             a[src] = np.nan
 
-    def record(self, u):
-        self.polls[u] = 0
+    def record(self, tick):
+        self.polls[tick] = 0
+        self.queued.append(tick)
+        assert sum(1 for t in self.queued if self.polls[t] < self.latency) <= self.depth
 
-    def done(self, u):
-        self.polls[u] += 1
-        return self.polls[u] >= self.latency
+    def done(self, tick):
+        self.polls[tick] += 1
+        return self.polls[tick] >= self.latency
 
-    def result(self, u, j):
-        return self.staged[u][j].copy()
+    def result(self, h):
+        return self.staged.pop(h)
 
     def close(self):
         pass
@@ -340,14 +344,18 @@ def test_step_slot_lanes_join_and_leave_between_steps():
 
 def test_step_slot_rejects_overfull_ticks():
     S = pkg("serve")
-    s = S.StepSlot(10, 16, max_batch=2, dev=HostLanes(5, 2))
+    s = S.StepSlot(10, 16, max_batch=2, dev=HostLanes(9, 2, depth=3))
+    assert s.depth == 3
     rs = [S.Request(i, 10, 16, np.ones(3), 0.0) for i in range(3)]
     with pytest.raises(ValueError):
         s.tick(rs)
     s.tick(rs[:2])
     s.tick()
-    with pytest.raises(RuntimeError):  # DEPTH ticks already queued
+    s.tick()
+    with pytest.raises(RuntimeError):  # depth ticks already queued
         s.tick()
+    # the queue depth stays below the outer degree, so a lane never starts and finishes within the queued ticks
+    assert S.StepSlot(10, 16, max_batch=2, dev=HostLanes(3, 2, depth=8)).depth == 2
 
 
 def test_poisson_trace_and_stats():
