@@ -153,9 +153,10 @@ class GpuSlot:
 
 class StepLanes:
     """The device side of a StepSlot: a resident plan whose ``max_batch`` lanes hold sinograms at different outer
-    steps, two iterate buffers and the lanes' sinograms on the device; page-locked rings of ``max_batch`` sinograms
-    and images for the uploads and downloads; an event per queued tick.  A ring entry is reused once the tick that
-    used it has completed (the slot waits for that tick in the rare case it has not).  The plan is a hybrid plan made
+    steps, two iterate buffers and the lanes' sinograms on the device; page-locked rings of 2 max_batch sinograms
+    (made on the first upload that needs staging) and max_batch images for the uploads and downloads; an event per
+    queued tick.  A sinogram entry is reused once the tick that uploaded from it has completed (the slot waits for
+    that tick in the rare case it has not).  The plan is a hybrid plan made
     from ``setup``, or any plan (hybrid or L_p) given as ``plan``."""
 
     def __init__(self, n, cap, precision, refine, max_batch, setup, streams, depth, plan=None, **kw):
@@ -181,9 +182,9 @@ class StepLanes:
         # at most 2 max_batch sinograms join within the queued ticks (the lanes, and as many again after lanes
         # finish) and at most max_batch images await reading (a lane cannot start and finish within them)
         self.nb = 2 * max_batch
-        self.host_b = L.PinnedArray((self.nb, 4, 2 * N - 1, N))
+        self.host_b = None
         self.host_x = L.PinnedArray((max_batch, N, N))
-        self.b_row = self.host_b.nbytes // self.nb
+        self.b_row = 4 * (2 * N - 1) * N * 8
         self.x_row = self.host_x.nbytes // max_batch
         self.b_owner = [-1] * self.nb  # tick that last uploaded from each ring entry
         self.x_unread = [False] * max_batch  # image entries downloaded into and not yet read
@@ -207,10 +208,12 @@ class StepLanes:
         if b_ptr is not None:  # page-locked already: uploaded from where it is
             L.check(L.load().adrt_memcpy(self.d_b.ptr + lane * self.b_row, b_ptr, self.b_row, 1, st.handle), "h2d")
             return
+        if self.host_b is None:
+            self.host_b = L.PinnedArray((self.nb, self.b_row // 8))
         e = self.b_next
         self.b_next = (e + 1) % self.nb
         self._free(self.b_owner[e])
-        self.host_b.a[e] = b
+        self.host_b.a[e] = np.asarray(b, dtype=np.float64).reshape(-1)
         self.b_owner[e] = tick
         L.check(L.load().adrt_memcpy(self.d_b.ptr + lane * self.b_row, self.host_b.ptr + e * self.b_row, self.b_row,
                                      1, st.handle), "h2d")
@@ -257,7 +260,8 @@ class StepLanes:
         self.streams.compute.sync()
         if self.own_plan:
             self.plan.close()
-        self.host_b.free()
+        if self.host_b is not None:
+            self.host_b.free()
         self.host_x.free()
         if self.own_streams:
             self.streams.close()
