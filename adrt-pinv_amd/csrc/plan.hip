@@ -140,13 +140,14 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         A.korder = p->korder_nd;
         A.nk = p->L - p->d.n_dense;
         A.skip_dense = 1;
-        // The MFMA tile (16 rows) pays off once several sinograms share W.  Every pass of an FP32 plan takes it (its
-        // FP64 passes too): its per-row arithmetic does not depend on the batch size, so a sinogram gives the same
-        // bits alone and in a batch.  (A last-bit difference of an FP64 pass would otherwise reach an FP32 rounding
-        // of the residual, whose cancellation magnifies it, and grow to the FP32 level.)  The register row kernel
-        // takes the MFMA tiles in extra waves of its own launch (the tiles use the matrix cores while the rows use
-        // the vector ALUs); otherwise they are a launch of their own.
-        const bool mfma = p->d.n >= 4 && (p->d.precision == PREC_F32 || 2 * S > 4);
+        // Every pass from N = 16 on takes the FP64 MFMA tiles, one sinogram included: in extra waves of the register
+        // row kernel's launch they stream W far faster than the vector GEMV of one block per frequency (one FP64
+        // sinogram at N=256: 59 -> 25 us per pass, 3.90 -> 2.51 ms per inverse).  A tile's per-row arithmetic does
+        // not depend on the batch size, so a sinogram gives the same bits alone and in a batch.  (In an FP32 plan a
+        // last-bit difference of an FP64 pass would otherwise reach an FP32 rounding of the residual, whose
+        // cancellation magnifies it, and grow to the FP32 level.)  Without the register row kernel the tiles are a
+        // launch of their own.
+        const bool mfma = p->d.n >= 4;
         int dense_done = 0;
         if (mfma && A.nk > 0) {
             A.dn.dense_k = p->dense_k;
