@@ -122,14 +122,14 @@ __global__ __launch_bounds__(256, 2) void dense_mfma_kernel(int n, DenseArgs D, 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = (by * wr + wave % wr) * 16 * RT;
     const int c0 = (bx * (4 / wr) + wave / wr) * 16 * CT;
-    if (r0 >= D.SP || c0 >= N) return;
+    if (r0 >= dense::unit_rows(D) || c0 >= N) return;
     dense::tile<Tg, U, RT, CT>(n, D, slot, r0, c0, lane, Yin, Zout);
 }
 
 template <typename Tg, int U, int RT, int CT>
 int launch_mfma_rc(int n, const DenseArgs& D, const void* Y, void* Z, hipStream_t st) {
     const int N = 1 << n;
-    const int rb = (D.SP + 16 * RT - 1) / (16 * RT);    // row blocks
+    const int rb = (dense::unit_rows(D) + 16 * RT - 1) / (16 * RT);  // row blocks
     const int cb = (N / 16 + CT - 1) / CT;              // output blocks
     const int wr = rb >= 4 ? 4 : (rb >= 2 ? 2 : 1);     // waves of a block along the rows
     const int gx = (cb + 4 / wr - 1) / (4 / wr), gy = (rb + wr - 1) / wr;
@@ -142,13 +142,17 @@ int launch_mfma_rc(int n, const DenseArgs& D, const void* Y, void* Z, hipStream_
 
 template <typename Tg>
 int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                hipStream_t st) {
+                const int* pair_k, const int* wslot, hipStream_t st) {
     DenseArgs D;
     D.dense_k = dense_k;
+    D.pair_k = pair_k;
+    D.wslot = wslot;
     D.WT = WT;
     D.n_dense = n_dense;
     D.L = L;
     D.SP = SP;
+    D.rows = pair_k ? 2 * SP : SP;
+    SP = D.rows;
     // blocked waves beyond one row tile; a single row tile (one or a few sinograms, where W streams from HBM) keeps
     // 16 x 16 tiles, twice the waves, with deeper load groups (8 k-steps for complex64 rows; complex128 rows take 4:
     // the larger group's registers would cost half the waves)
@@ -158,14 +162,15 @@ int launch_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c12
 }
 }  // namespace
 
-// Z_r = W_slot Y_r on the FP64 matrix cores (requires N >= 16 and Z distinct from Y).
+// Z_r = W_slot Y_r on the FP64 matrix cores (requires N >= 16 and Z distinct from Y); with pair_k, units of
+// conjugate frequency pairs (DenseArgs).
 int launch_dense_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                      int prec, hipStream_t st) {
+                      int prec, hipStream_t st, const int* pair_k, const int* wslot) {
     if (n_dense <= 0 || SP <= 0) return OK;
     ADRT_REQUIRE(n >= 4 && n <= 11, "dense_mfma: need 4 <= n <= 11");
     ADRT_REQUIRE(Z != Y, "dense_mfma: the output must not alias the input rows");
-    if (prec == PREC_F64) return launch_mfma<double>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
-    return launch_mfma<float>(n, L, SP, n_dense, dense_k, WT, Y, Z, st);
+    if (prec == PREC_F64) return launch_mfma<double>(n, L, SP, n_dense, dense_k, WT, Y, Z, pair_k, wslot, st);
+    return launch_mfma<float>(n, L, SP, n_dense, dense_k, WT, Y, Z, pair_k, wslot, st);
 }
 
 // Z_r = W_slot Y_r for the SP rows of every dense frequency (rows sp * L + k); Z may alias Y.

@@ -57,6 +57,14 @@ struct Plan {
     int* korder = nullptr;  // frequencies by descending degree (block order of the row kernel)
     int* korder_nd = nullptr;  // the same without the dense frequencies
     int* dense_k = nullptr;    // dense frequencies (GEMM slots)
+    // the GEMM's units (dense::CONJ marks a frequency applied with the conjugate of its partner's W): one frequency
+    // each (unit_k, unit_w; n_units), or conjugate pairs sharing one W (pair_k1, pair_k2, pair_w; n_punits)
+    int* unit_k = nullptr;
+    int* unit_w = nullptr;
+    int* pair_k1 = nullptr;
+    int* pair_k2 = nullptr;
+    int* pair_w = nullptr;
+    int n_units = 0, n_punits = 0, n_pairs = 0;
     c128* WopT = nullptr;      // (n_dense, N, N) pass matrices, transposed
     c128* WbackT = nullptr;
     void* tw = nullptr;    // FFT twiddles exp(-2 pi i m / L), m < N, double
@@ -85,13 +93,16 @@ struct Plan {
 
 namespace {
 
+// rows per frequency (2 per sinogram) up to which conjugate dense frequencies share one unit
+constexpr int DENSE_PAIR_ROWS = 32;
+
 size_t row_elem(int prec) { return prec == PREC_F64 ? sizeof(c128) : sizeof(c64); }
 size_t strip_elem(int prec) { return prec == PREC_F64 ? sizeof(double) : sizeof(float); }
 
 void free_plan(Plan* p) {
     if (!p) return;
     if (p->borrowed) p->dFT = p->WopT = p->WbackT = nullptr;
-    void* bufs[] = {p->z, p->deg, p->a, p->c, p->wtab, p->ztab64, p->ztab32, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
+    void* bufs[] = {p->z, p->deg, p->a, p->c, p->wtab, p->ztab64, p->ztab32, p->dslot, p->dFT, p->korder, p->korder_nd, p->dense_k, p->unit_k, p->unit_w, p->pair_k1, p->pair_k2, p->pair_w, p->WopT, p->WbackT, p->tw, p->tw32, p->Y, p->Y2, p->Sq, p->y, p->xs, p->res, p->r, p->g};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -149,12 +160,18 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         // launch of their own.
         const bool mfma = p->d.n >= 4;
         int dense_done = 0;
+        // conjugate pairs share one stream of W where W dominates (few rows per frequency); with many rows the
+        // frequencies go alone (N=256, 128 sinograms: 495 against 509 us per row launch).  Same bits either way.
+        const bool pairs = p->n_pairs > 0 && 2 * S <= DENSE_PAIR_ROWS;
         if (mfma && A.nk > 0) {
-            A.dn.dense_k = p->dense_k;
+            A.dn.dense_k = pairs ? p->pair_k1 : p->unit_k;
+            A.dn.pair_k = pairs ? p->pair_k2 : nullptr;
+            A.dn.wslot = pairs ? p->pair_w : p->unit_w;
             A.dn.WT = WT;
-            A.dn.n_dense = p->d.n_dense;
+            A.dn.n_dense = pairs ? p->n_punits : p->n_units;
             A.dn.L = p->L;
             A.dn.SP = 2 * S;
+            A.dn.rows = pairs ? 4 * S : 2 * S;
         }
         if (A.nk > 0) {
             timing_events(p, A);
@@ -163,7 +180,13 @@ int rows_pass(Plan* p, int prec, const RowOp* ops, int nops, int S, hipStream_t 
         if (dense_done) return OK;
         // the dense GEMM's row type: complex128 when the row kernel took complex128 rows
         const int rp = io64 ? PREC_F64 : prec;
-        if (mfma) return launch_dense_mfma(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, rp, st);
+        if (mfma) {
+            if (pairs)
+                return launch_dense_mfma(p->d.n, p->L, 2 * S, p->n_punits, p->pair_k1, WT, p->Y, A.Z, rp, st,
+                                         p->pair_k2, p->pair_w);
+            return launch_dense_mfma(p->d.n, p->L, 2 * S, p->n_units, p->unit_k, WT, p->Y, A.Z, rp, st, nullptr,
+                                     p->unit_w);
+        }
         return launch_dense_gemm(p->d.n, p->L, 2 * S, p->d.n_dense, p->dense_k, WT, p->Y, A.Z, rp, st);
     }
     timing_events(p, A);
@@ -385,6 +408,41 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
             return fail(st);
         if ((st = alloc((void**)&p->dense_k, sizeof(int) * d->n_dense)) ||
             (st = up(p->dense_k, d->dense_k, sizeof(int) * d->n_dense)))
+            return fail(st);
+        // Conjugate partners: k and L - k when their twiddles are exact conjugates (z from the angles 2 pi
+        // fftfreq(2N) always are).  Every op of a pass has real coefficients, so the partner's pass matrix is the
+        // conjugate of k's: the partner's rows always run as conj(conj(y) W_k), alone or in one unit with k's rows,
+        // where one stream of W_k serves both.  Its own table is never read.
+        std::vector<int> uk, uw, pk1, pk2, pw;
+        std::vector<char> used((size_t)d->n_dense, 0);
+        std::vector<int> slot_of(L, -1);
+        for (int i = 0; i < d->n_dense; ++i) slot_of[d->dense_k[i]] = i;
+        for (int i = 0; i < d->n_dense; ++i) {
+            if (used[i]) continue;
+            used[i] = 1;
+            const int k = d->dense_k[i], kp = (L - k) % L;
+            const int j = slot_of[kp];
+            const bool conj = j >= 0 && j != i && !used[j] && d->z[kp].re == d->z[k].re && d->z[kp].im == -d->z[k].im;
+            uk.push_back(k);
+            uw.push_back(i);
+            pk1.push_back(k);
+            pw.push_back(i);
+            pk2.push_back(conj ? (kp | dense::CONJ) : -1);
+            if (conj) {
+                used[j] = 1;
+                uk.push_back(kp | dense::CONJ);
+                uw.push_back(i);
+            }
+        }
+        p->n_units = (int)uk.size();
+        p->n_punits = (int)pk1.size();
+        p->n_pairs = p->n_units - p->n_punits;
+        const size_t ub = sizeof(int) * uk.size(), pb = sizeof(int) * pk1.size();
+        if ((st = alloc((void**)&p->unit_k, ub)) || (st = up(p->unit_k, uk.data(), ub)) ||
+            (st = alloc((void**)&p->unit_w, ub)) || (st = up(p->unit_w, uw.data(), ub)) ||
+            (st = alloc((void**)&p->pair_k1, pb)) || (st = up(p->pair_k1, pk1.data(), pb)) ||
+            (st = alloc((void**)&p->pair_k2, pb)) || (st = up(p->pair_k2, pk2.data(), pb)) ||
+            (st = alloc((void**)&p->pair_w, pb)) || (st = up(p->pair_w, pw.data(), pb)))
             return fail(st);
     }
     p->d.dense_Wop = p->d.dense_Wback = nullptr;

@@ -105,8 +105,9 @@ int dense_config_set(int n, int prec, int mode);
 int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
                       int prec, hipStream_t st);
 
+// The same on the FP64 matrix cores; with pair_k, n_dense units of conjugate frequency pairs (DenseArgs).
 int launch_dense_mfma(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,
-                      int prec, hipStream_t st);
+                      int prec, hipStream_t st, const int* pair_k = nullptr, const int* wslot = nullptr);
 
 // Validates an op list against n; returns OK or E_INVALID with the error set.
 int check_ops(int n, const RowOp* ops, int nops);

@@ -211,6 +211,30 @@ def test_dense_placement_same_bits(gpu, prec, n, S):
         H.close()
 
 
+@pytest.mark.parametrize("prec,n,S", [("fp64", 8, 20), ("fp32", 8, 20), ("fp64", 10, 17)])
+def test_conjugate_dense_pairs_same_bits(gpu, prec, n, S):
+    """Dense frequencies k and L - k have conjugate pass matrices; the partner's rows always run as
+    conj(conj(y) W_k).  With few rows per frequency (one sinogram) the two share one unit and one stream of W_k,
+    with many (20 or 17 sinograms: 40 or 34 rows) each has a unit of its own.  A sinogram gives the same bits either
+    way, and the FP64 result stays within 1e-12 of the plan that reads every table (the oracle tests of the FP64
+    inverses pin that)."""
+    I = pkg("inverses")
+    C = pkg("plancache")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    N = 1 << n
+    lam, outer = C.hybrid_setup(n, dense_cap=16)
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(900 + i, N)), 0.01, 950 + i) for i in range(S)])
+    H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=16, precision=prec, max_batch=S, refine=1)
+    try:
+        assert H.eng.n_dense > 1
+        x = H.apply_batch(B)
+        for i in (0, S - 1):
+            assert np.array_equal(x[i], H.apply(B[i])), i
+    finally:
+        H.close()
+
+
 @pytest.mark.parametrize("n,loge,S", [(10, 3, 8), (10, 2, 8), (8, 3, 64)])
 def test_row_groups_per_block_same_bits(gpu, n, loge, S):
     """Blocks of float rows run several row groups of their frequency on one set-up once the batch gives the launch
