@@ -44,12 +44,22 @@ def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
+def dense_tables_read(H):
+    """Dense pass matrices a pass reads: frequencies k and L - k with exactly conjugate twiddles share k's matrix (the
+    plan applies its conjugate to the partner), so a pair reads one."""
+    ks = {int(k) for k in H.eng.dense_k}
+    z = np.asarray(H.eng.z)
+    Lr = len(z)
+    pairs = sum(1 for k in ks if (Lr - k) % Lr in ks and (Lr - k) % Lr > k and z[(Lr - k) % Lr] == np.conj(z[k]))
+    return len(ks) - pairs
+
+
 def dense_work(H, S):
     """FP64 MFMA FLOPs and HBM bytes of the dense tiles in one row-kernel launch over S sinograms: per dense frequency
-    the 2S rows times the N x N complex pass matrix (8 N^2 flops per row), the matrix read once (complex128) and the
-    rows read and written once (complex64)."""
+    the 2S rows times the N x N complex pass matrix (8 N^2 flops per row), each pass matrix read once (complex128; a
+    conjugate pair of frequencies reads one) and the rows read and written once (complex64)."""
     N, nd, SP = H.N, H.eng.n_dense, 2 * S
-    return 8.0 * nd * SP * N * N, nd * N * N * 16.0 + 2.0 * nd * SP * N * 8
+    return 8.0 * nd * SP * N * N, dense_tables_read(H) * N * N * 16.0 + 2.0 * nd * SP * N * 8
 
 
 def rows_kernel_work(H, S, ops):
