@@ -148,7 +148,9 @@ typedef struct {
     int max_batch;      /* sinograms per call; scratch is sized for it */
     /* optional: the whole op list of a dense row as one matrix (row-major (n_dense, N, N)): the operator pass
        (B1^H M' F B1 for G_h, M F for the frame) and the backprojection pass (B1^H F A'^H, F A^H).  When given,
-       dense rows run as a batched GEMM instead of inside the row kernel. */
+       dense rows run as a batched GEMM instead of inside the row kernel.  For dense frequencies k and L - k whose z
+       are exact conjugates, the GEMM applies the conjugate of k's matrices to the rows of L - k (the ops have real
+       coefficients, so these are L - k's matrices) and does not read L - k's own. */
     const adrt_c128* dense_Wop;
     const adrt_c128* dense_Wback;
     int refine;         /* ADRT_PREC_F32 only: the backprojection and the last `refine` outer steps run in FP64 */
