@@ -41,17 +41,17 @@ constexpr int FREQ_MASK = CONJ - 1;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// The block of RT x CT tiles (rows r0 .. r0 + 16 RT - 1, outputs c0 .. c0 + 16 CT - 1) of dense slot `slot`, computed
-// by one wave (lane 0..63): each k-step loads RT row fragments and CT W fragments and runs 3 RT CT MFMAs, so a W
-// fragment serves RT row tiles and a row fragment CT output tiles.  U k-steps (of 4) per load group, the next group's
-// loads in flight while the current group's MFMAs run.  Every output element accumulates the same k-steps in the same
-// order for any RT, CT and U, so the bits do not depend on the blocking.  Zout must not alias Yin (other waves still
-// read the frequency's rows).
 // Sign flip of x where the high word of `mask` has its top bit set (exact; one integer op).
 __device__ __forceinline__ double flip(double x, unsigned mask) {
     return __hiloint2double(__double2hiint(x) ^ (int)mask, __double2loint(x));
 }
 
+// The block of RT x CT tiles (rows r0 .. r0 + 16 RT - 1, outputs c0 .. c0 + 16 CT - 1) of dense unit `slot`, computed
+// by one wave (lane 0..63): each k-step loads RT row fragments and CT W fragments and runs 3 RT CT MFMAs, so a W
+// fragment serves RT row tiles and a row fragment CT output tiles.  U k-steps (of 4) per load group, the next group's
+// loads in flight while the current group's MFMAs run.  Every output element accumulates the same k-steps in the same
+// order for any RT, CT and U, so the bits do not depend on the blocking.  Zout must not alias Yin (other waves still
+// read the frequency's rows).
 template <typename Tg, int U, int RT, int CT>
 __device__ __forceinline__ void tile(int n, const DenseArgs& D, int slot, int r0, int c0, int lane, const void* Yin,
                                      void* Zout) {
