@@ -103,7 +103,9 @@ def test_tick_rejects_bad_lane_orders(gpu):
 def test_server_continuous_keys_n512_n1024(gpu):
     """The server with N = 512 and 1024 as continuous keys (requests join between outer steps) beside batched
     N = 128: every result equal, bit for bit, to its slot's plan applied to it alone, and within 1e-5 of the FP64
-    inverse with the same setup."""
+    inverse with the same setup.  Every other request arrives in page-locked memory (Request.b_ptr) and uploads from
+    there without staging, in both kinds of slot."""
+    L = pkg("lib")
     S = pkg("serve")
     C = pkg("plancache")
     I = pkg("inverses")
@@ -114,8 +116,17 @@ def test_server_continuous_keys_n512_n1024(gpu):
     srv = S.Server(keys, slots_per_key=1, max_batch={7: 4, 9: 4, 10: 4}, setups=setups, refine=refine,
                    continuous={9, 10})
     assert sum(1 for s in srv.slots if getattr(s, "continuous", False)) == 3
-    reqs = [S.Request(i, *keys[i % len(keys)], _sino(keys[i % len(keys)][0], 700 + i, 800 + i), 0.0)
-            for i in range(14)]
+    reqs, pinned = [], []
+    for i in range(14):
+        n, p = keys[i % len(keys)]
+        b = _sino(n, 700 + i, 800 + i)
+        if i % 2:
+            pb = L.PinnedArray(b.shape)
+            pb.a[...] = b
+            pinned.append(pb)
+            reqs.append(S.Request(i, n, p, pb.a, 0.0, b_ptr=pb.ptr))
+        else:
+            reqs.append(S.Request(i, n, p, b, 0.0))
     trace = [(0.004 * i, r) for i, r in enumerate(reqs)]
     done = srv.run_trace(trace)
     assert len(done) == len(reqs) and all(r.error is None for r in reqs)
@@ -129,3 +140,5 @@ def test_server_continuous_keys_n512_n1024(gpu):
         assert relerr(r.x, H64.apply(r.b)) < 1e-5, (n, p)
         H64.close()
     srv.close()
+    for pb in pinned:
+        pb.free()
