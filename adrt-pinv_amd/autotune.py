@@ -49,7 +49,8 @@ def pick_dense(times):
 
 class TuneResult(tuple):
     """(loge, {loge: ms}); the dense placement chosen after it is in .dense, its timings in .dense_times; for an FP32
-    plan with FP64 refinement passes, their row mapping in .f64 (None otherwise), its timings in .f64_times."""
+    plan with FP64 refinement passes, their row mapping in .f64 (None otherwise), its timings in .f64_times; the dense
+    placement of one-sinogram passes in .dense1 (None when not tuned), its timings in .dense1_times."""
 
     def __new__(cls, loge, times, dense=0, dense_times=None, f64=None, f64_times=None):
         r = super().__new__(cls, (loge, times))
@@ -57,6 +58,7 @@ class TuneResult(tuple):
         r.dense_times = dense_times or {}
         r.f64 = f64
         r.f64_times = f64_times or {}
+        r.dense1, r.dense1_times = None, {}
         return r
 
 
@@ -110,8 +112,9 @@ def _time_pass(lib, plan, S, st, e0, e1, reps, which=1):
 
 def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=DENSE_CANDIDATES, persist=True):
     """Time the row pass of ``plan`` (a FastLpHip / FastHybridHip with S <= max_batch sinograms) for every candidate
-    mapping, then (plans with dense rows) every dense placement at the chosen mapping; install both and return
-    TuneResult(best_loge, {loge: ms}) with .dense and .dense_times."""
+    mapping, then (plans with dense rows) every dense placement at the chosen mapping, and with S > 1 again for passes
+    over one sinogram (dense class precision + 2); install them and return TuneResult(best_loge, {loge: ms}) with
+    .dense, .dense_times, .dense1 and .dense1_times."""
     lib = L.load()
     prec = L.CONFIG_PRECISION[plan.precision]
     n = plan.n
@@ -139,6 +142,17 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=D
             dense_times[m] = _time_pass(lib, plan, S, st, e0, e1, reps)
         dense = pick_dense(dense_times)
     lib.adrt_dense_config_set(n, prec, dense)
+    # passes over one sinogram (dense class prec + 2), where the W stream dominates: N=1024 at one sinogram takes
+    # 299 us per row pass with two dense waves per SIMD against 337 us with one, while with more sinograms one wins
+    dense1, dense1_times = None, {}
+    if getattr(plan.eng, "n_dense", 0) > 0 and best != -1 and S > 1:
+        plan.operator_device(dx, dy, 1, st)
+        for m in dense_candidates:
+            L.check(lib.adrt_dense_config_set(n, prec + 2, m), "dense_config_set")
+            dense1_times[m] = _time_pass(lib, plan, 1, st, e0, e1, reps)
+        dense1 = pick_dense(dense1_times)
+        lib.adrt_dense_config_set(n, prec + 2, dense1)
+        plan.operator_device(dx, dy, S, st)
     # the FP64 refinement passes of an FP32 plan (class 2): an apply leaves the rows of its last, FP64 pass
     f64, f64_times = None, {}
     if plan.precision != "fp64" and getattr(plan, "refine", 0) > 0:
@@ -158,7 +172,9 @@ def tune(plan, S, stream=None, reps=3, candidates=CANDIDATES, dense_candidates=D
         if f64 is not None:
             table[(n, 2)] = (f64, 0)
         save_table(table)
-    return TuneResult(best, times, dense, dense_times, f64, f64_times)
+    res = TuneResult(best, times, dense, dense_times, f64, f64_times)
+    res.dense1, res.dense1_times = dense1, dense1_times
+    return res
 
 
 REFINE_CHOICES = (1, 2, 3, 4, 6, 8, 12)

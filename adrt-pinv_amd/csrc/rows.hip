@@ -304,15 +304,20 @@ int rows_config_set(int n, int prec, int loge) {
     return OK;
 }
 
-static int g_dense_cfg[2][12];
+// classes 0 / 1: FP64 / FP32 passes; 2 / 3: the same with one sinogram (two rows per frequency), where the W stream
+// dominates and more dense waves can pay (0 there: as class 0 / 1)
+static int g_dense_cfg[4][12];
 
 int dense_config_get(int n, int prec) {
-    if (n < 0 || n > 11 || prec < 0 || prec > 1) return 0;
+    if (n < 0 || n > 11 || prec < 0 || prec > 3) return 0;
+    if (prec >= 2 && g_dense_cfg[prec][n] == 0) return g_dense_cfg[prec - 2][n];
     return g_dense_cfg[prec][n];
 }
 
+int dense_class(int prec, const DenseArgs& dn) { return prec + (dn.SP <= 2 ? 2 : 0); }
+
 int dense_config_set(int n, int prec, int mode) {
-    ADRT_REQUIRE(n >= 0 && n <= 11 && (prec == 0 || prec == 1), "dense_config_set: bad n or precision");
+    ADRT_REQUIRE(n >= 0 && n <= 11 && prec >= 0 && prec <= 3, "dense_config_set: bad n or class");
     ADRT_REQUIRE(mode >= -1 && mode <= 8, "dense_config_set: mode must be -1 .. 8");
     g_dense_cfg[prec][n] = mode;
     return OK;
@@ -342,7 +347,7 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
         const int cfg = rows_config_get(A.n, cls);
         const int loge = A.loge > 0 ? A.loge : (cfg > 0 ? cfg : default_loge(A.n, prec));
         ADRT_REQUIRE(loge > 0, "rows: complex128 rows with a float core need 32 <= N <= 2048");
-        const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, prec) >= 0;
+        const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, dense_class(prec, A.dn)) >= 0;
         RowsArgs R = A;
         if (!fuse) R.dn = DenseArgs{};
         const int st = launch_rows_reg(R, prec, loge, s);
@@ -354,7 +359,7 @@ int launch_rows(const RowsArgs& A, int prec, hipStream_t s, int* dense_done) {
         const int cfg = rows_config_get(A.n, cls);
         const int loge = A.loge > 0 ? A.loge : (cfg != 0 ? cfg : default_loge(A.n, prec));
         if (loge > 0) {
-            const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, prec) >= 0;
+            const bool fuse = A.dn.n_dense > 0 && dense_config_get(A.n, dense_class(prec, A.dn)) >= 0;
             RowsArgs R = A;
             if (!fuse) R.dn = DenseArgs{};
             const int st = launch_rows_reg(R, prec, loge, s);

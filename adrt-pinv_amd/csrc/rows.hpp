@@ -97,9 +97,11 @@ int cheb_table_fill(int K, const int* deg_host, const int* d_deg, const double* 
                     hipStream_t st, const c128** wcoef, const int** wcoff);
 int rows_config_get(int n, int prec);
 int rows_config_set(int n, int prec, int loge);
-// dense GEMM placement per (n, prec): 0 default, -1 own launch, k > 0 at most k dense waves per SIMD in the row launch
+// dense GEMM placement per (n, class): 0 default, -1 own launch, k > 0 at most k dense waves per SIMD in the row
+// launch; class = precision (0 FP64, 1 FP32), plus 2 for passes of one sinogram (unset: as the precision's)
 int dense_config_get(int n, int prec);
 int dense_config_set(int n, int prec, int mode);
+int dense_class(int prec, const DenseArgs& dn);
 
 // Dense rows as a GEMM with the per-frequency pass matrices W^T (dense.hip).
 int launch_dense_gemm(int n, int L, int SP, int n_dense, const int* dense_k, const c128* WT, const void* Y, void* Z,

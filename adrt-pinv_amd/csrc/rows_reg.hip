@@ -61,7 +61,7 @@ int dense_workers(int n, int prec, const DenseArgs& dn) {
         sm = 4 * cus;
         simds[dev].store(sm, std::memory_order_relaxed);
     }
-    const int per = dense_config_get(n, prec);
+    const int per = dense_config_get(n, dense_class(prec, dn));
     if (per > 1) sm *= per;
     const long long T = dense::tiles(n, dn);
     return (int)(T < sm ? T : sm);

@@ -468,7 +468,8 @@ def main():
     log(f"autotuned row mapping: loge={loge} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tune_ms.items())}); "
         f"dense placement {tuned.dense} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.dense_times.items())}); "
         f"FP64 refinement passes loge={tuned.f64} "
-        f"({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.f64_times.items())})")
+        f"({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.f64_times.items())}); one sinogram: dense placement "
+        f"{tuned.dense1} ({', '.join(f'{k}: {v * 1e3:.0f} us' for k, v in tuned.dense1_times.items())})")
 
     # inputs: ellipse phantoms (seeds 1000+i) through the GPU forward ADRT, plus 1% noise (seeds 2000+i)
     Bn = make_sinos(n, B, 1000, 2000)

@@ -109,7 +109,8 @@ int adrt_rows_config_get(int n, int precision);
 
 /* Dense-row GEMM placement per (n, precision) for passes whose dense rows take the MFMA path: 0 = built-in default
  * (one dense wave per SIMD inside the register row kernel's launch), -1 = a launch of its own after the rows,
- * 1..8 = at most that many dense waves per SIMD inside the row kernel's launch.  Chosen by the autotuner.  (No
+ * 1..8 = at most that many dense waves per SIMD inside the row kernel's launch.  precision 2 / 3: the placement of
+ * FP64 / FP32 passes over one sinogram (unset or 0: that of precision 0 / 1).  Chosen by the autotuner.  (No
  * counterpart in the reference, whose dense rows run inside its row loop.) */
 int adrt_dense_config_set(int n, int precision, int mode);
 int adrt_dense_config_get(int n, int precision);

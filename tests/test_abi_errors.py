@@ -77,9 +77,11 @@ def test_memory_and_plan_calls_reject(L):
 
 
 def test_dense_config_validates(L):
+    """Classes 0 / 1 (FP64 / FP32 passes) and 2 / 3 (the same over one sinogram, following 0 / 1 until set)."""
     lib = L.load()
     assert lib.adrt_dense_config_set(12, 0, 1) == -1 and "n" in err(L)
-    assert lib.adrt_dense_config_set(8, 2, 1) == -1 and "precision" in err(L)
+    assert lib.adrt_dense_config_set(8, 4, 1) == -1 and "class" in err(L)
+    assert lib.adrt_dense_config_set(8, -1, 1) == -1 and "class" in err(L)
     assert lib.adrt_dense_config_set(8, 1, 9) == -1 and "mode" in err(L)
     assert lib.adrt_dense_config_set(8, 1, -2) == -1
     old = lib.adrt_dense_config_get(8, 1)
@@ -87,8 +89,14 @@ def test_dense_config_validates(L):
         for m in (-1, 0, 1, 2, 4, 8):
             assert lib.adrt_dense_config_set(8, 1, m) == 0
             assert lib.adrt_dense_config_get(8, 1) == m
+            assert lib.adrt_dense_config_get(8, 3) == m  # one sinogram: as FP32 until set
+        assert lib.adrt_dense_config_set(8, 3, 4) == 0
+        assert lib.adrt_dense_config_get(8, 3) == 4 and lib.adrt_dense_config_get(8, 1) == 8
+        assert lib.adrt_dense_config_set(8, 3, 0) == 0
+        assert lib.adrt_dense_config_get(8, 3) == 8
     finally:
         lib.adrt_dense_config_set(8, 1, old)
+        lib.adrt_dense_config_set(8, 3, 0)
 
 
 def test_rows_config_classes(L):

@@ -211,6 +211,30 @@ def test_dense_placement_same_bits(gpu, prec, n, S):
         H.close()
 
 
+@pytest.mark.parametrize("prec,n", [("fp32", 10), ("fp64", 8)])
+def test_one_sinogram_dense_placement_same_bits(gpu, prec, n):
+    """Passes over one sinogram take the dense placement of their own class (precision + 2, tuned apart because the W
+    stream dominates there); every placement gives the same bits, equal to the sinogram's result in a batch."""
+    I = pkg("inverses")
+    C = pkg("plancache")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    lib = gpu.load()
+    N = 1 << n
+    lam, outer = C.hybrid_setup(n, dense_cap=16)
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(960 + i, N)), 0.01, 970 + i) for i in range(3)])
+    H = I.FastHybridHip(n, lam_max=lam, outer=outer, dense_cap=16, precision=prec, max_batch=3, refine=1)
+    cls = gpu.CONFIG_PRECISION[prec] + 2
+    try:
+        xb = H.apply_batch(B)
+        for m in (-1, 1, 2, 4):
+            gpu.check(lib.adrt_dense_config_set(n, cls, m), "dense_config_set")
+            assert np.array_equal(H.apply(B[1]), xb[1]), m
+    finally:
+        lib.adrt_dense_config_set(n, cls, 0)
+        H.close()
+
+
 @pytest.mark.parametrize("prec,n,S", [("fp64", 8, 20), ("fp32", 8, 20), ("fp64", 10, 17)])
 def test_conjugate_dense_pairs_same_bits(gpu, prec, n, S):
     """Dense frequencies k and L - k have conjugate pass matrices; the partner's rows always run as
