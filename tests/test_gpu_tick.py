@@ -3,8 +3,9 @@
 Sinograms join a running batch between outer steps and leave after their last one.  Each result must be, bit for
 bit, the plan's whole apply of that sinogram alone: the tick runs the apply's arithmetic step for step, per lane.
 Covered: the FP32 hybrid plan with its FP64 refinement step (three kinds of step in one tick), the FP64 hybrid plan,
-the lorr preconditioner and a G_1 degree above the fused kernel's halo (the unfused update, per lane), L_p, and the
-server with continuous keys at N = 512 and 1024 against FP64.
+the lorr preconditioner and a G_1 degree above the fused kernel's halo (the unfused update, per lane), the explicit
+level and G_1 in FP64 inside FP32 steps, FP32 without TwoSum or refinement, L_p, and the server with continuous keys
+at N = 512 and 1024 against FP64.
 """
 
 import numpy as np
@@ -53,6 +54,8 @@ def run_staggered(plan, B, starts):
     (6, dict(precision="fp64")),
     (6, dict(precision="fp64", precond="lorr")),
     (6, dict(precision="fp32", refine=1, g1_deg=20)),
+    (7, dict(precision="fp32", refine=1, dense_cap=8, l1_fp64=True, g1_fp64=True)),
+    (7, dict(precision="fp32_plain", refine=0, dense_cap=8)),
 ])
 def test_tick_lanes_same_bits_as_apply(gpu, n, kw):
     """Seven sinograms on four lanes, joining at ticks 0, 0, 2, 3, 3 and later as lanes free: lanes at the plan's
