@@ -432,6 +432,9 @@ def main():
                          "empty: no second leg")
     ap.add_argument("--serve-lanes", default=SERVE_LANES,
                     help="lanes of the continuous sizes as n:lanes, comma separated")
+    ap.add_argument("--serve-batch", default=None,
+                    help="sinograms per launch of the batched sizes as n:batch, comma separated (default: "
+                         + ",".join(f"{k}:{v}" for k, v in SERVE_BATCH.items()) + ")")
     ap.add_argument("--serve-tick-depth", type=int, default=None,
                     help="ticks kept queued per continuous slot (default 16)")
     ap.add_argument("--rows-only", action="store_true")
@@ -558,6 +561,8 @@ def main():
     serving = serving_cont = None
     if not a.no_serve:
         rates = [float(r) for r in a.serve_rates.split(",")]
+        if a.serve_batch:
+            SERVE_BATCH.update({int(k): int(v) for k, v in (e.split(":") for e in a.serve_batch.split(",") if e)})
         serving = serving_leg(rates, a.serve_duration, max_wait_s=a.serve_max_wait)
         cont = [int(v) for v in a.serve_continuous.split(",") if v.strip()]
         if cont:
