@@ -171,6 +171,30 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const
     }
 }
 
+// The combine of the paired planes (launch_ifft_pair): out[a][c] = U[N-1-c][a] + V[a][c] (or y - that), U read
+// through a transposed LDS tile.
+template <typename Tg, typename To>
+__global__ __launch_bounds__(256) void combine_pair_kernel(int n, const Tg* UV, const double* y, To* out) {
+    __shared__ double tu[32][33];
+    const int N = 1 << n;
+    const int s = blockIdx.z;
+    const int a0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const Tg* U = UV + (size_t)s * 2 * N * N;
+    const Tg* V = U + (size_t)N * N;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    // tu[c][a] = U[N-1-(c0+c)][a0+a]: contiguous along a
+    for (int r = ty; r < 32; r += 8)
+        if (c0 + r < N && a0 + tx < N) tu[r][tx] = (double)U[(size_t)(N - 1 - (c0 + r)) * N + a0 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int a = a0 + r, c = c0 + tx;
+        if (a >= N || c >= N) continue;
+        const double v = tu[tx][r] + (double)V[(size_t)a * N + c];
+        const size_t o = ((size_t)s * N + a) * N + c;
+        out[o] = (To)(y ? y[o] - v : v);
+    }
+}
+
 int fft_cols(int n, size_t elem) {
     const size_t L = (size_t)2 << n;
     const size_t N = (size_t)1 << n;
@@ -248,6 +272,27 @@ int launch_combine(int n, const void* Sq, int prec, const double* y, void* out, 
         hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(256), 0, st, n, (const float*)Sq, y,
                            (float*)out);
     return launch_check("combine_kernel");
+}
+
+int launch_combine_pair(int n, const void* UV, int prec, const double* y, void* out, int S, hipStream_t st,
+                        bool out_f32) {
+    ADRT_REQUIRE(n >= 5 && n <= 11, "combine_pair: need 5 <= n <= 11");
+    if (S <= 0) return OK;
+    const int T = (1 << n) / 32;
+    const dim3 grid(T, T, S);
+    if (prec == PREC_F64 && !out_f32)
+        hipLaunchKernelGGL((combine_pair_kernel<double, double>), grid, dim3(256), 0, st, n, (const double*)UV, y,
+                           (double*)out);
+    else if (prec == PREC_F64)
+        hipLaunchKernelGGL((combine_pair_kernel<double, float>), grid, dim3(256), 0, st, n, (const double*)UV, y,
+                           (float*)out);
+    else if (!out_f32)
+        hipLaunchKernelGGL((combine_pair_kernel<float, double>), grid, dim3(256), 0, st, n, (const float*)UV, y,
+                           (double*)out);
+    else
+        hipLaunchKernelGGL((combine_pair_kernel<float, float>), grid, dim3(256), 0, st, n, (const float*)UV, y,
+                           (float*)out);
+    return launch_check("combine_pair_kernel");
 }
 
 }  // namespace adrt

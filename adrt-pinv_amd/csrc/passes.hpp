@@ -24,6 +24,17 @@ int launch_combine(int n, const void* Sq, int prec, const double* y, void* out, 
 // Two-step register FFT for 6 <= n <= 9 (fft2.hip); mode 0 / 1 as launch_pack_fft, mode 2 as launch_ifft_unpack.
 bool fft2_supported(int n);
 int launch_fft2(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st);
+// The inverse FFT with the four strips summed in the two pairs that need no transpose between them (fft2.hip):
+//   UV[s, 0] = U, U[h][a] = S0[h][a] + S3[h][N-1-a];   UV[s, 1] = V, V[a][c] = S1[N-1-a][c] + S2[a][c]
+// (each strip value rounded to float first, as launch_ifft_unpack stores it, and the sum rounded to float): half the
+// strip traffic of launch_ifft_unpack + launch_combine.  A block runs four column FFTs in turn, so it has a quarter of
+// launch_ifft_unpack's blocks: ifft_pair_blocks(n, prec) per sinogram, 0 where unsupported (float, 7 <= n <= 9).
+// launch_combine_pair then adds U^T + V.
+int ifft_pair_blocks(int n, int prec);
+int launch_ifft_pair(int n, const void* Z, void* UV, int S, int prec, const void* tw, hipStream_t st);
+// out[s] = U^T + V windowed to N x N: out[a][c] = U[N-1-c][a] + V[a][c], or y - that; types as launch_combine.
+int launch_combine_pair(int n, const void* UV, int prec, const double* y, void* out, int S, hipStream_t st,
+                        bool out_f32 = false);
 // Three-step register FFT for n = 10, 11 (fft3.hip), modes as launch_fft2.
 bool fft3_supported(int n);
 int launch_fft3(int n, int mode, const void* src, void* dst, int S, int prec, const void* tw, hipStream_t st);

@@ -93,6 +93,10 @@ struct Plan {
 
 namespace {
 
+// blocks of the paired inverse FFT at a plan's max_batch from which its FP32 passes take it: two per CU.  With fewer
+// its four column FFTs per block in turn cost more latency than the halved traffic saves (measured with a double
+// variant on the FP64 path at one sinogram, N=256: 2.53 -> 3.18 ms per inverse).
+constexpr long long PAIR_IFFT_MIN_BLOCKS = 512;
 // rows per frequency (2 per sinogram) up to which conjugate dense frequencies share one unit
 constexpr int DENSE_PAIR_ROWS = 32;
 
@@ -203,6 +207,13 @@ int packed_pass(Plan* p, int prec, int mode, const double* src, const RowOp* ops
     const void* tw = pf == PREC_F64 ? p->tw : p->tw32;
     ADRT_TRY(launch_pack_fft(n, mode, src, p->Y, S, pf, tw, st));
     ADRT_TRY(rows_pass(p, prec, ops, nops, S, st, p->Y2, mode == 1 ? p->WbackT : p->WopT, l1_64));
+    // FP32 passes of plans whose full batch gives the paired inverse two blocks per CU take it (two planes instead of
+    // four strips); a plan decides by its max_batch, never by the batch of a call, so an image's bits do not depend on
+    // how many others share the launch
+    if ((long long)p->d.max_batch * ifft_pair_blocks(n, pf) >= PAIR_IFFT_MIN_BLOCKS) {
+        ADRT_TRY(launch_ifft_pair(n, p->Y2, p->Sq, S, pf, tw, st));
+        return launch_combine_pair(n, p->Sq, pf, resid_y, out, S, st, out_f32);
+    }
     ADRT_TRY(launch_ifft_unpack(n, p->Y2, p->Sq, S, pf, tw, st));
     return launch_combine(n, p->Sq, pf, resid_y, out, S, st, out_f32);
 }

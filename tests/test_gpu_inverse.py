@@ -180,6 +180,38 @@ def test_fp32_batch_headline_config(gpu):
     assert np.array_equal(x32[2], H32.apply(B[2]))
 
 
+@pytest.mark.parametrize("n,big", [(7, 256), (8, 64), (9, 32)])
+def test_fp32_paired_inverse_fft(gpu, n, big):
+    """FP32 plans whose max_batch gives the paired inverse FFT two blocks per CU take it: it writes U = S0 + S3 (the
+    transposed strips, mirrored columns) and V = S1 + S2 (mirrored rows) instead of the four quadrant strips, and the
+    combine adds U^T + V.  Against a plan of the same setup at max_batch 4 (four strips): the operator G_h and the
+    inverse differ in bits but agree to FP32 rounding (a strip summed at the wrong row or column would not), both are
+    within 1e-5 of FP64, and the big plan gives a sinogram the same bits alone and in a batch."""
+    def relerr(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    N = 1 << n
+    B = np.stack([ph.add_noise(R.adrt(ph.ellipses(1100 + i, N)), 0.01, 2100 + i) for i in range(3)])
+    x = ph.ellipses(1200, N)
+    H64 = I.FastHybridHip(n, dense_cap=16, max_batch=3)
+    kw = dict(lam_max=H64.lam_max, dense_cap=16, precision="fp32", outer=(H64.lo, H64.hi, H64.outer_deg), refine=1)
+    Hb = I.FastHybridHip(n, max_batch=big, **kw)
+    Hs = I.FastHybridHip(n, max_batch=4, **kw)
+    g64, gb, gs = H64.Gh(x), Hb.Gh(x), Hs.Gh(x)
+    assert not np.array_equal(gb, gs)
+    assert relerr(gb, gs) < 1e-6 and relerr(gb, g64) < 1e-5 and relerr(gs, g64) < 1e-5
+    x64, xb, xs = H64.apply_batch(B), Hb.apply_batch(B), Hs.apply_batch(B)
+    for i in range(3):
+        assert relerr(xb[i], x64[i]) < 1e-5 and relerr(xs[i], x64[i]) < 1e-5, i
+        assert relerr(xb[i], xs[i]) < 2e-6, i
+    assert np.array_equal(Hb.apply(B[1]), xb[1])
+    for P in (H64, Hb, Hs):
+        P.close()
+
+
 @pytest.mark.parametrize("prec,n,S", [("fp32", 8, 3), ("fp64", 8, 3), ("fp32", 9, 3), ("fp32", 8, 24),
                                        ("fp64", 8, 12), ("fp32", 7, 9)])
 def test_dense_placement_same_bits(gpu, prec, n, S):
