@@ -151,6 +151,16 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const
     const Tg* S2 = S1 + (size_t)N * N;
     const Tg* S3 = S2 + (size_t)N * N;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    // the untransposed strips and y first, so that their loads are in flight across the tile's barrier
+    double v1[4], v2[4], yv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int a = a0 + ty + 8 * i, c = c0 + tx;
+        const bool in = a < N && c < N;
+        v1[i] = in ? (double)S1[(size_t)(N - 1 - a) * N + c] : 0.0;
+        v2[i] = in ? (double)S2[(size_t)a * N + c] : 0.0;
+        yv[i] = (in && y) ? y[((size_t)s * N + a) * N + c] : 0.0;
+    }
     // t0[c][a] = S0[N-1-(c0+c)][a0+a], t3[c][a] = S3[N-1-(c0+c)][N-1-(a0+a)]: contiguous along a
     for (int r = ty; r < 32; r += 8) {
         if (c0 + r < N && a0 + tx < N) {
@@ -160,14 +170,13 @@ __global__ __launch_bounds__(256) void combine_kernel(int n, const Tg* Sq, const
         }
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int a = a0 + r, c = c0 + tx;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = ty + 8 * i, a = a0 + r, c = c0 + tx;
         if (a >= N || c >= N) continue;
-        const double v1 = (double)S1[(size_t)(N - 1 - a) * N + c];
-        const double v2 = (double)S2[(size_t)a * N + c];
-        const double v = (t0[tx][r] + v1) + (v2 + t3[tx][r]);
+        const double v = (t0[tx][r] + v1[i]) + (v2[i] + t3[tx][r]);
         const size_t o = ((size_t)s * N + a) * N + c;
-        out[o] = (To)(y ? y[o] - v : v);
+        out[o] = (To)(y ? yv[i] - v : v);
     }
 }
 
@@ -182,16 +191,26 @@ __global__ __launch_bounds__(256) void combine_pair_kernel(int n, const Tg* UV, 
     const Tg* U = UV + (size_t)s * 2 * N * N;
     const Tg* V = U + (size_t)N * N;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    // V and y first, so that their loads are in flight across the tile's barrier
+    double vv[4], yv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int a = a0 + ty + 8 * i, c = c0 + tx;
+        const bool in = a < N && c < N;
+        vv[i] = in ? (double)V[(size_t)a * N + c] : 0.0;
+        yv[i] = (in && y) ? y[((size_t)s * N + a) * N + c] : 0.0;
+    }
     // tu[c][a] = U[N-1-(c0+c)][a0+a]: contiguous along a
     for (int r = ty; r < 32; r += 8)
         if (c0 + r < N && a0 + tx < N) tu[r][tx] = (double)U[(size_t)(N - 1 - (c0 + r)) * N + a0 + tx];
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int a = a0 + r, c = c0 + tx;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = ty + 8 * i, a = a0 + r, c = c0 + tx;
         if (a >= N || c >= N) continue;
-        const double v = tu[tx][r] + (double)V[(size_t)a * N + c];
+        const double v = tu[tx][r] + vv[i];
         const size_t o = ((size_t)s * N + a) * N + c;
-        out[o] = (To)(y ? y[o] - v : v);
+        out[o] = (To)(y ? yv[i] - v : v);
     }
 }
 

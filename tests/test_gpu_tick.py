@@ -4,8 +4,8 @@ Sinograms join a running batch between outer steps and leave after their last on
 bit, the plan's whole apply of that sinogram alone: the tick runs the apply's arithmetic step for step, per lane.
 Covered: the FP32 hybrid plan with its FP64 refinement step (three kinds of step in one tick), the FP64 hybrid plan,
 the lorr preconditioner and a G_1 degree above the fused kernel's halo (the unfused update, per lane), the explicit
-level and G_1 in FP64 inside FP32 steps, FP32 without TwoSum or refinement, L_p, and the server with continuous keys
-at N = 512 and 1024 against FP64.
+level and G_1 in FP64 inside FP32 steps, FP32 without TwoSum or refinement, a plan whose max_batch gives its FP32
+passes the paired inverse FFT (N = 256, 64), L_p, and the server with continuous keys at N = 512 and 1024 against FP64.
 """
 
 import numpy as np
@@ -56,13 +56,14 @@ def run_staggered(plan, B, starts):
     (6, dict(precision="fp32", refine=1, g1_deg=20)),
     (7, dict(precision="fp32", refine=1, dense_cap=8, l1_fp64=True, g1_fp64=True)),
     (7, dict(precision="fp32_plain", refine=0, dense_cap=8)),
+    (8, dict(precision="fp32", refine=1, dense_cap=16, max_batch=64)),
 ])
 def test_tick_lanes_same_bits_as_apply(gpu, n, kw):
     """Seven sinograms on four lanes, joining at ticks 0, 0, 2, 3, 3 and later as lanes free: lanes at the plan's
     own steps, at FP64 refinement steps and at step 0 share ticks, lanes leave and move down, and every result is
     the plan's apply of that sinogram alone, bit for bit."""
     I = pkg("inverses")
-    P = I.FastHybridHip(n, max_batch=4, **kw)
+    P = I.FastHybridHip(n, **{"max_batch": 4, **kw})
     D = P.outer_deg
     B = [_sino(n, 300 + i, 400 + i) for i in range(7)]
     starts = [0, 0, 2, 3, 3, D - 1, D]
