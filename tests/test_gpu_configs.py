@@ -48,7 +48,7 @@ def _oracle_hybrid(n, H, cap):
 def test_n256_single_sinogram_fp64_vs_reference_engine(gpu, cap):
     """N=256, ellipse phantom seed 1 + 1% noise seed 2, dense_cap p: the FP64 GPU hybrid's G_h, backprojection and
     whole inverse within 1e-12 of the reference engine on the same setup scalars; the FP32 plan (TwoSum rows, one
-    FP64 refinement step) within 1e-5 of it."""
+    FP64 refinement step) within 1e-5 of it, at max_batch 1 and at the headline's 128 (the paired inverse FFT)."""
     I = pkg("inverses")
     ph = pkg("phantom")
     n = 8
@@ -65,6 +65,11 @@ def test_n256_single_sinogram_fp64_vs_reference_engine(gpu, cap):
     H32 = I.FastHybridHip(n, lam_max=H.lam_max, dense_cap=cap, precision="fp32", refine=1,
                           outer=(H.lo, H.hi, H.outer_deg))
     assert relerr(H32.apply(b), ref) < 1e-5
+    # the headline's plan shape: max_batch 128 gives its FP32 passes the paired inverse FFT
+    H32p = I.FastHybridHip(n, lam_max=H.lam_max, dense_cap=cap, precision="fp32", refine=1, max_batch=128,
+                           outer=(H.lo, H.hi, H.outer_deg))
+    assert relerr(H32p.apply(b), ref) < 1e-5
+    H32p.close()
     H32.close()
     H.close()
 
