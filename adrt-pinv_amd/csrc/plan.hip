@@ -94,8 +94,8 @@ struct Plan {
 namespace {
 
 // blocks of the paired inverse FFT at a plan's max_batch from which its FP32 passes take it: two per CU.  With fewer
-// its four column FFTs per block in turn cost more latency than the halved traffic saves (measured with a double
-// variant on the FP64 path at one sinogram, N=256: 2.53 -> 3.18 ms per inverse).
+// its four column FFTs per block in turn can cost more latency than the halved traffic saves (whole applies, N=256:
+// one sinogram 2.91 -> 3.50 ms with it, 32 sinograms 9.50 -> 9.56 ms, 63 at max_batch 64 18.0 -> 17.4 ms).
 constexpr long long PAIR_IFFT_MIN_BLOCKS = 512;
 // rows per frequency (2 per sinogram) up to which conjugate dense frequencies share one unit
 constexpr int DENSE_PAIR_ROWS = 32;
