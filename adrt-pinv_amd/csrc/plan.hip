@@ -65,7 +65,7 @@ struct Plan {
     int* pair_k2 = nullptr;
     int* pair_w = nullptr;
     int n_units = 0, n_punits = 0, n_pairs = 0;
-    c128* WopT = nullptr;      // (n_dense, N, N) pass matrices, transposed
+    c128* WopT = nullptr;      // (tables, N, N) pass matrices, transposed
     c128* WbackT = nullptr;
     void* tw = nullptr;    // FFT twiddles exp(-2 pi i m / L), m < N, double
     void* tw32 = nullptr;  // the same rounded to float
@@ -254,6 +254,56 @@ int check_batch(Plan* p, int S) {
     return OK;
 }
 
+// The dense GEMM's units, formed on the host from the descriptor (dense::CONJ marks a frequency applied with the
+// conjugate of its partner's pass matrices): one frequency each (uk, uw) or conjugate pairs sharing one table
+// (pk1, pk2, pw), and the number of tables the descriptor's pass matrices hold.
+struct DenseUnits {
+    std::vector<int> uk, uw, pk1, pk2, pw;
+    int n_tables = 0;
+};
+
+int dense_units(const adrt_plan_desc* d, DenseUnits& U) {
+    const int L = 2 << d->n, nd = d->n_dense;
+    std::vector<int> slot_of(L, -1);
+    U.n_tables = d->dense_w ? 0 : nd;
+    for (int i = 0; i < nd; ++i) {
+        const int k = d->dense_k[i];
+        ADRT_REQUIRE(k >= 0 && k < L, "plan_create: dense_k out of range");
+        ADRT_REQUIRE(slot_of[k] < 0, "plan_create: dense_k lists a frequency twice");
+        slot_of[k] = i;
+        if (d->dense_w) {
+            ADRT_REQUIRE(d->dense_w[i] >= -1 && d->dense_w[i] < nd, "plan_create: dense_w out of range");
+            U.n_tables = std::max(U.n_tables, d->dense_w[i] + 1);
+        }
+    }
+    auto table = [&](int i) { return d->dense_w ? d->dense_w[i] : i; };
+    // Conjugate partners: k and L - k when their twiddles are exact conjugates (z from the angles 2 pi fftfreq(2N)
+    // always are).  Every op of a pass has real coefficients, so the partner's pass matrix is the conjugate of k's:
+    // the partner's rows always run as conj(conj(y) W_k), alone or in one unit with k's rows, where one stream of W_k
+    // serves both.  The partner's own table, if it has one, is never read.
+    std::vector<char> used((size_t)nd, 0);
+    for (int i = 0; i < nd; ++i) {
+        if (used[i] || table(i) < 0) continue;
+        used[i] = 1;
+        const int k = d->dense_k[i], kp = (L - k) % L;
+        const int j = slot_of[kp];
+        const bool conj = j >= 0 && j != i && !used[j] && d->z[kp].re == d->z[k].re && d->z[kp].im == -d->z[k].im;
+        U.uk.push_back(k);
+        U.uw.push_back(table(i));
+        U.pk1.push_back(k);
+        U.pw.push_back(table(i));
+        U.pk2.push_back(conj ? (kp | dense::CONJ) : -1);
+        if (conj) {
+            used[j] = 1;
+            U.uk.push_back(kp | dense::CONJ);
+            U.uw.push_back(table(i));
+        }
+    }
+    for (int i = 0; i < nd; ++i)
+        ADRT_REQUIRE(used[i], "plan_create: dense_w gives a frequency no pass matrices and no conjugate partner with them");
+    return OK;
+}
+
 }  // namespace
 
 }  // namespace adrt
@@ -275,8 +325,15 @@ unsigned long long adrt_plan_bytes(const adrt_plan_desc* d) {
         const std::vector<int> zdeg(L, 0);
         b += cheb_table_bytes((int)L, zdeg.data());
     }
-    if (d->dense_on_device != 2)
-        b += (size_t)d->n_dense * N * N * sizeof(c128) * ((d->dense_Wop && d->dense_Wback) ? 3 : 1);
+    if (d->dense_on_device != 2 && d->n_dense > 0) {
+        size_t nt = (size_t)d->n_dense;  // pass-matrix tables (max(dense_w) + 1 with dense_w)
+        if (d->dense_w && d->dense_k) {
+            nt = 0;
+            for (int i = 0; i < d->n_dense; ++i) nt = std::max(nt, (size_t)std::max(d->dense_w[i] + 1, 0));
+        }
+        b += ((d->dense_F ? (size_t)d->n_dense : 0) + ((d->dense_Wop && d->dense_Wback) ? 2 * nt : 0)) * N * N *
+             sizeof(c128);
+    }
     b += twiddle_bytes(d->n, PREC_F64) + twiddle_bytes(d->n, PREC_F32);
     b += 2 * S * 2 * L * N * row_elem(PREC_F64);
     b += S * 4 * N * N * strip_elem(PREC_F64);
@@ -307,7 +364,12 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
                  "plan_create: null row parameter array");
     ADRT_REQUIRE(d->kind != ADRT_KIND_NORMAL || d->n_dense == 0, "plan_create: the normal operator has no dense rows");
     ADRT_REQUIRE(d->n_dense >= 0 && d->n_dense <= (2 << d->n), "plan_create: bad n_dense");
-    ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && d->dense_F), "plan_create: dense rows need dense_k and dense_F");
+    const bool has_w = d->dense_Wop && d->dense_Wback;
+    ADRT_REQUIRE(d->n_dense == 0 || (d->dense_k && (d->dense_F || has_w)),
+                 "plan_create: dense rows need dense_k and dense_F or the pass matrices");
+    ADRT_REQUIRE(!d->dense_w || (has_w && d->n >= 4), "plan_create: dense_w needs the pass matrices and n >= 4");
+    DenseUnits U;
+    if (d->n_dense > 0) ADRT_TRY(dense_units(d, U));
     auto* p = new Plan();
     p->d = *d;
     p->comp = d->precision == ADRT_PREC_F32;
@@ -318,6 +380,7 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     p->d.a = p->d.c = nullptr;
     p->d.dense_k = nullptr;
     p->d.dense_F = nullptr;
+    p->d.dense_w = nullptr;
     const int n = d->n, N = 1 << n, L = 2 * N;
     p->N = N;
     p->L = L;
@@ -359,28 +422,22 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
     }
     if (d->n_dense > 0) {
         std::vector<int> slot(L, -1);
-        for (int i = 0; i < d->n_dense; ++i) {
-            if (d->dense_k[i] < 0 || d->dense_k[i] >= L) {
-                set_error("plan_create: dense_k out of range");
-                return fail(E_INVALID);
-            }
-            slot[d->dense_k[i]] = i;
-        }
+        for (int i = 0; i < d->n_dense; ++i) slot[d->dense_k[i]] = i;
         if ((st = alloc((void**)&p->dslot, sizeof(int) * L)) || (st = up(p->dslot, slot.data(), sizeof(int) * L)))
             return fail(st);
         const size_t tb = sizeof(c128) * (size_t)d->n_dense * N * N;
         if (d->dense_on_device == 2) {
             p->borrowed = true;
             p->dFT = (c128*)d->dense_F;
-        } else if ((st = alloc((void**)&p->dFT, tb))) {
+        } else if (d->dense_F && (st = alloc((void**)&p->dFT, tb))) {
             return fail(st);
         }
-        if (d->dense_on_device == 1) {
+        if (d->dense_F && d->dense_on_device == 1) {
             if (hipMemcpy(p->dFT, d->dense_F, tb, hipMemcpyDeviceToDevice) != hipSuccess) {
                 set_error("plan_create: device copy of the dense tables failed");
                 return fail(E_HIP);
             }
-        } else if (d->dense_on_device == 0) {
+        } else if (d->dense_F && d->dense_on_device == 0) {
             std::vector<c128> ft((size_t)d->n_dense * N * N);
             const c128* F = (const c128*)d->dense_F;
             for (int s = 0; s < d->n_dense; ++s)
@@ -390,26 +447,26 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
             if ((st = up(p->dFT, ft.data(), tb))) return fail(st);
         }
     }
-    // pass matrices of the dense rows (stored transposed for coalesced streaming)
+    // pass matrices of the dense rows (stored transposed for coalesced streaming): U.n_tables of them
     if (d->n_dense > 0 && d->dense_Wop && d->dense_Wback) {
-        const size_t nn = (size_t)N * N;
-        std::vector<c128> t(d->dense_on_device ? 0 : (size_t)d->n_dense * nn);
+        const size_t nn = (size_t)N * N, nt = (size_t)U.n_tables;
+        std::vector<c128> t(d->dense_on_device ? 0 : nt * nn);
         auto transpose_up = [&](const adrt_c128* W, c128** dst) -> int {
             if (d->dense_on_device == 2) {
                 *dst = (c128*)W;
                 return OK;
             }
             if (d->dense_on_device) {
-                int r = alloc((void**)dst, sizeof(c128) * d->n_dense * nn);
+                int r = alloc((void**)dst, sizeof(c128) * nt * nn);
                 if (r) return r;
-                if (hipMemcpy(*dst, W, sizeof(c128) * d->n_dense * nn, hipMemcpyDeviceToDevice) != hipSuccess) {
+                if (hipMemcpy(*dst, W, sizeof(c128) * nt * nn, hipMemcpyDeviceToDevice) != hipSuccess) {
                     set_error("plan_create: device copy of the pass matrices failed");
                     return E_HIP;
                 }
                 return OK;
             }
             const c128* w = (const c128*)W;
-            for (int s = 0; s < d->n_dense; ++s)
+            for (size_t s = 0; s < nt; ++s)
                 for (int i = 0; i < N; ++i)
                     for (int j = 0; j < N; ++j) t[s * nn + (size_t)j * N + i] = w[s * nn + (size_t)i * N + j];
             int r = alloc((void**)dst, sizeof(c128) * t.size());
@@ -420,40 +477,15 @@ int adrt_plan_create(adrt_plan** out, const adrt_plan_desc* d) {
         if ((st = alloc((void**)&p->dense_k, sizeof(int) * d->n_dense)) ||
             (st = up(p->dense_k, d->dense_k, sizeof(int) * d->n_dense)))
             return fail(st);
-        // Conjugate partners: k and L - k when their twiddles are exact conjugates (z from the angles 2 pi
-        // fftfreq(2N) always are).  Every op of a pass has real coefficients, so the partner's pass matrix is the
-        // conjugate of k's: the partner's rows always run as conj(conj(y) W_k), alone or in one unit with k's rows,
-        // where one stream of W_k serves both.  Its own table is never read.
-        std::vector<int> uk, uw, pk1, pk2, pw;
-        std::vector<char> used((size_t)d->n_dense, 0);
-        std::vector<int> slot_of(L, -1);
-        for (int i = 0; i < d->n_dense; ++i) slot_of[d->dense_k[i]] = i;
-        for (int i = 0; i < d->n_dense; ++i) {
-            if (used[i]) continue;
-            used[i] = 1;
-            const int k = d->dense_k[i], kp = (L - k) % L;
-            const int j = slot_of[kp];
-            const bool conj = j >= 0 && j != i && !used[j] && d->z[kp].re == d->z[k].re && d->z[kp].im == -d->z[k].im;
-            uk.push_back(k);
-            uw.push_back(i);
-            pk1.push_back(k);
-            pw.push_back(i);
-            pk2.push_back(conj ? (kp | dense::CONJ) : -1);
-            if (conj) {
-                used[j] = 1;
-                uk.push_back(kp | dense::CONJ);
-                uw.push_back(i);
-            }
-        }
-        p->n_units = (int)uk.size();
-        p->n_punits = (int)pk1.size();
+        p->n_units = (int)U.uk.size();
+        p->n_punits = (int)U.pk1.size();
         p->n_pairs = p->n_units - p->n_punits;
-        const size_t ub = sizeof(int) * uk.size(), pb = sizeof(int) * pk1.size();
-        if ((st = alloc((void**)&p->unit_k, ub)) || (st = up(p->unit_k, uk.data(), ub)) ||
-            (st = alloc((void**)&p->unit_w, ub)) || (st = up(p->unit_w, uw.data(), ub)) ||
-            (st = alloc((void**)&p->pair_k1, pb)) || (st = up(p->pair_k1, pk1.data(), pb)) ||
-            (st = alloc((void**)&p->pair_k2, pb)) || (st = up(p->pair_k2, pk2.data(), pb)) ||
-            (st = alloc((void**)&p->pair_w, pb)) || (st = up(p->pair_w, pw.data(), pb)))
+        const size_t ub = sizeof(int) * U.uk.size(), pb = sizeof(int) * U.pk1.size();
+        if ((st = alloc((void**)&p->unit_k, ub)) || (st = up(p->unit_k, U.uk.data(), ub)) ||
+            (st = alloc((void**)&p->unit_w, ub)) || (st = up(p->unit_w, U.uw.data(), ub)) ||
+            (st = alloc((void**)&p->pair_k1, pb)) || (st = up(p->pair_k1, U.pk1.data(), pb)) ||
+            (st = alloc((void**)&p->pair_k2, pb)) || (st = up(p->pair_k2, U.pk2.data(), pb)) ||
+            (st = alloc((void**)&p->pair_w, pb)) || (st = up(p->pair_w, U.pw.data(), pb)))
             return fail(st);
     }
     p->d.dense_Wop = p->d.dense_Wback = nullptr;

@@ -12,7 +12,8 @@ transpose the plan stores.  The level matrices come from the package's own butte
 rows it returns are the columns of A, i.e. A in column-major order.
 
 Tables are cached per (kind, n, tau, dense frequencies) for the life of the process (``clear_tables`` drops the
-cache), and plans borrow them, so every plan of one size shares one copy in HBM.
+cache), and plans borrow them, so every plan of one size shares one copy in HBM.  Only the pass matrices stay, and
+only one per conjugate pair of frequencies (``table_slots``): a third of the F, W_op, W_back of every frequency.
 """
 
 import ctypes
@@ -105,7 +106,7 @@ def clear_tables():
 
 
 def dense_tables(n, z, ks, tau, kind):
-    """(F, W_op, W_back) for the dense frequencies ks, from the cache or built by ``dense_setup``."""
+    """(F, W_op, W_back, w) for the dense frequencies ks, from the cache or built by ``dense_setup``."""
     key = (kind, int(n), float(tau), tuple(int(k) for k in ks))
     if key not in _TABLES:
         _TABLES[key] = dense_setup(n, z, ks, tau, kind)
@@ -113,36 +114,67 @@ def dense_tables(n, z, ks, tau, kind):
 
 
 def table_bytes(tables):
-    return sum(t.nbytes for t in tables)
+    return sum(t.nbytes for t in tables if t is not None)
 
 
-def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
-    """Device arrays (K, N, N) of F_k, W_op,k and W_back,k in column-major order for the dense frequencies ks."""
+def table_slots(z, ks):
+    """Table index of each dense frequency (adrt_plan_desc.dense_w).  Frequencies k and L - k whose twiddles are exact
+    conjugates have conjugate pass matrices (every op has real coefficients), and the plan applies the first one's
+    conjugated to the other: the first of such a pair in the order of ks, and every frequency without a partner, get
+    the next index; the partner gets -1.  This is the plan's own pairing rule (adrt_plan_create), so every table built
+    is read."""
+    z = np.asarray(z)
+    L = z.size
+    slot = {int(k): i for i, k in enumerate(ks)}
+    w = np.full(len(ks), -1, dtype=np.int32)
+    used = np.zeros(len(ks), dtype=bool)
+    t = 0
+    for i, k in enumerate(int(k) for k in ks):
+        if used[i]:
+            continue
+        used[i] = True
+        w[i] = t
+        t += 1
+        kp = (L - k) % L
+        j = slot.get(kp, -1)
+        if j >= 0 and j != i and not used[j] and z[kp].real == z[k].real and z[kp].imag == -z[k].imag:
+            used[j] = True
+    return w
+
+
+def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30, keep_F=False):
+    """Device arrays of W_op,k and W_back,k in column-major order, and w, the table index of each dense frequency
+    (``table_slots``; None below n = 4, where every frequency has its own table).  Only frequencies with w >= 0 get
+    tables, in the order of w.  F_k, an intermediate here (device plans never read it once they have the pass
+    matrices), is returned for the same frequencies with keep_F, else None."""
     N = 1 << n
     ks = np.asarray(ks, dtype=np.int64)
-    K = ks.size
+    w = table_slots(z, ks) if n >= 4 else None
+    own = ks if w is None else ks[w >= 0]
+    K = own.size
     R = RocLinalg.get()
     lo = 2 if kind == "hybrid" else 1
-    F = L.DeviceArray((K, N, N), np.complex128)
+    F = L.DeviceArray((K, N, N), np.complex128) if keep_F else None
     Wop = L.DeviceArray((K, N, N), np.complex128)
     Wback = L.DeviceArray((K, N, N), np.complex128)
     per = max(1, int(chunk_bytes // (7 * N * N * 16)))
     nbytes = N * N * 16
     lib = L.load()
     for i0 in range(0, K, per):
-        kk = ks[i0 : i0 + per]
+        kk = own[i0 : i0 + per]
         b = kk.size
         off = i0 * nbytes
         Ap = _level_device(n, z, kk, lo, n)
         M = L.DeviceArray((b, N, N), np.complex128)
         T1 = L.DeviceArray((b, N, N), np.complex128)
         T2 = L.DeviceArray((b, N, N), np.complex128)
+        Ftmp = None if keep_F else L.DeviceArray((b, N, N), np.complex128)
         info = L.DeviceArray((b,), np.int32)
         R.gemm(_OP_C, _OP_N, N, Ap.ptr, Ap.ptr, M.ptr, b)  # M = A^H A
         L.check(lib.adrt_memcpy(T2.ptr, M.ptr, M.nbytes, 3, None), "d2d")
         R.add_diag(N, T2.ptr, tau, b)  # M + tau I
         R.potrf(N, T2.ptr, info.ptr, b)
-        Fc = F.ptr + off
+        Fc = F.ptr + off if keep_F else Ftmp.ptr
         L.check(lib.adrt_memset(Fc, 0, b * nbytes, None), "memset")
         R.add_diag(N, Fc, 1.0, b)  # I
         R.potrs(N, T2.ptr, Fc, b)  # F = (M + tau I)^{-1}
@@ -164,7 +196,7 @@ def dense_setup(n, z, ks, tau, kind, chunk_bytes=1 << 30):
             R.gemm(_OP_N, _OP_N, N, M.ptr, Fc, Wop.ptr + off, b)  # M F
             R.gemm(_OP_N, _OP_C, N, Fc, Ap.ptr, Wback.ptr + off, b)  # F A^H
         L.check(lib.adrt_device_sync(), "sync")
-    return F, Wop, Wback
+    return F, Wop, Wback, w
 
 
 def to_row_major(dev):

@@ -54,15 +54,15 @@ class _PlanBase:
     def _make_plan(self, lo=1.0, hi=2.0, D=1):
         e = self.eng
         self._keep = [np.ascontiguousarray(e.z), np.ascontiguousarray(e.deg, dtype=np.int32), e.a, e.c]
-        Fd = Wop = Wback = None
+        Fd = Wop = Wback = w = None
         self.table_bytes = 0
         if e.n_dense:
             # dense tables and pass matrices built on the GPU (rocSOLVER Cholesky + rocBLAS GEMMs), column-major on
             # the device, shared by every plan of this size (the plan borrows them; _keep holds them alive)
             kind = "hybrid" if self.KIND == L.KIND_HYBRID else "lp"
-            Fd, Wop, Wback = G.dense_tables(self.n, self.z, e.dense_k, self.tau, kind)
+            Fd, Wop, Wback, w = G.dense_tables(self.n, self.z, e.dense_k, self.tau, kind)
             self.table_bytes = G.table_bytes((Fd, Wop, Wback))
-            self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), Fd, Wop, Wback]
+            self._keep += [np.ascontiguousarray(e.dense_k, dtype=np.int32), Fd, Wop, Wback, w]
         d = L.PlanDesc()
         d.kind = self.KIND
         d.n = self.n
@@ -83,6 +83,7 @@ class _PlanBase:
         d.dense_Wback = L.ptr(Wback)
         d.refine = int(self.refine)
         d.dense_on_device = 2
+        d.dense_w = L.ptr(w)
         d.g1_fp64 = int(bool(getattr(self, "g1_fp64", False)))
         d.l1_fp64 = int(bool(getattr(self, "l1_fp64", False)))
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))

@@ -109,7 +109,7 @@ class PlanDesc(ctypes.Structure):
         ("lam_max", P), ("z", P), ("deg", P), ("a", P), ("c", P),
         ("n_dense", I), ("dense_k", P), ("dense_F", P), ("max_batch", I),
         ("dense_Wop", P), ("dense_Wback", P), ("refine", I), ("dense_on_device", I), ("g1_fp64", I),
-        ("l1_fp64", I),
+        ("l1_fp64", I), ("dense_w", P),
     ]
 
 

@@ -145,7 +145,8 @@ typedef struct {
     const double* c;    /* (L) */
     int n_dense;        /* frequencies with the exact filter */
     const int* dense_k; /* (n_dense) their indices in [0, L) */
-    const adrt_c128* dense_F; /* (n_dense, N, N) row-major */
+    const adrt_c128* dense_F; /* (n_dense, N, N) row-major; may be NULL when dense_Wop and dense_Wback are given
+                                 (plans with pass matrices never read it) */
     int max_batch;      /* sinograms per call; scratch is sized for it */
     /* optional: the whole op list of a dense row as one matrix (row-major (n_dense, N, N)): the operator pass
        (B1^H M' F B1 for G_h, M F for the frame) and the backprojection pass (B1^H F A'^H, F A^H).  When given,
@@ -164,6 +165,11 @@ typedef struct {
     int l1_fp64;        /* ADRT_PREC_F32 hybrid plans: 1 runs the explicit level 1 (B 1..1, BT 1..1 of fastops.py:330), the
                            offset FFTs and the combine of the FP32 outer steps in double; the row kernel then reads and
                            writes complex128 rows and runs levels 2..n and the Chebyshev filter in float (default 0) */
+    const int* dense_w; /* optional (n_dense), n >= 4, with dense_Wop / dense_Wback: the index of dense frequency i's
+                           pass matrices in those tables, or -1 when it has none of its own; then L - k is dense, its
+                           z the exact conjugate of k's, with an index >= 0, and the plan applies that partner's
+                           matrices conjugated.  The tables hold max(dense_w) + 1 matrices.  NULL: frequency i's
+                           matrices are entry i of n_dense */
 } adrt_plan_desc;
 
 typedef struct adrt_plan adrt_plan;

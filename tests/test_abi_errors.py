@@ -180,3 +180,46 @@ def test_noalloc_forward_rejects_before_device_work(L):
     assert lib.adrt_forward_noalloc(5, 3, None, None, 1, None) == -1 and "dtype" in err(L)
     assert lib.adrt_forward_noalloc(5, 0, None, None, -1, None) == -1 and "S" in err(L)
     assert lib.adrt_forward_noalloc(5, 0, None, None, 1, None) == -1 and "null" in err(L)
+
+
+def _dense_desc(L, n, ks, w, with_w=True):
+    N = 1 << n
+    z = np.exp(1j * 2 * np.pi * np.fft.fftfreq(2 * N))
+    d, keep = desc(L, n=n, z=None)
+    d.z = L.ptr(z)
+    kk = np.asarray(ks, dtype=np.int32)
+    ww = None if w is None else np.asarray(w, dtype=np.int32)
+    d.n_dense, d.dense_k, d.dense_w = len(ks), L.ptr(kk), L.ptr(ww)
+    if with_w:
+        d.dense_Wop = d.dense_Wback = 4096  # never dereferenced: the descriptor is rejected first
+    return d, (keep, z, kk, ww)
+
+
+@pytest.mark.parametrize("n,ks,w,with_w,word", [
+    (4, [1, 31, 5], [0, -1, -1], True, "no conjugate partner"),  # 5's partner 27 is not dense
+    (4, [1, 31], [0, 2], True, "dense_w out of range"),
+    (4, [1, 31], [-1, -1], True, "no conjugate partner"),
+    (3, [1, 15], [0, -1], True, "n >= 4"),
+    (4, [1, 31], [0, -1], False, "pass matrices"),
+    (4, [1, 1], None, True, "twice"),
+])
+def test_plan_create_rejects_dense_tables(L, n, ks, w, with_w, word):
+    d, keep = _dense_desc(L, n, ks, w, with_w)
+    h = L.P()
+    assert L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)) == -1
+    assert h.value is None and word in err(L)
+
+
+def test_table_slots_pairs_conjugate_frequencies():
+    """One pass-matrix table per conjugate pair (k, L - k), the first in the dense list's order; self-conjugate
+    frequencies (0 and N) and frequencies whose partner is not dense keep their own."""
+    G = pkg("gpusetup")
+    N = 32
+    z = np.exp(1j * 2 * np.pi * np.fft.fftfreq(2 * N))
+    assert list(G.table_slots(z, [0, 1, 2, 62, 63])) == [0, 1, 2, -1, -1]
+    assert list(G.table_slots(z, [63, 32, 1, 5])) == [0, 1, -1, 2]
+    # twiddles that are not exact conjugates are not paired
+    z2 = z.copy()
+    z2[63] = np.conj(z[1]) * (1 + 1e-16j)
+    z2[63] = complex(z2[63].real, np.nextafter(z2[63].imag, 0.0))
+    assert list(G.table_slots(z2, [1, 63])) == [0, 1]

@@ -106,9 +106,54 @@ def test_gpu_dense_setup_matches_host(gpu, kind):
     ks = np.array([0, 1, 2, 62, 63], dtype=np.int32)
     tau = 0.2 * N
     lo = 2 if kind == "hybrid" else 1
-    F, Wop, Wback = G.dense_setup(n, z, ks, tau, kind)
+    F, Wop, Wback, w = G.dense_setup(n, z, ks, tau, kind, keep_F=True)
     Fh = E.dense_tables(n, z, tau, ks, lo, n)
     Wo, Wb = E.pass_matrices(n, z, ks, Fh, kind)
-    assert relmax(G.to_row_major(F), Fh) < 1e-12
-    assert relmax(G.to_row_major(Wop), Wo) < 1e-12
-    assert relmax(G.to_row_major(Wback), Wb) < 1e-12
+    # one table per conjugate pair: 62 = L - 2 and 63 = L - 1 take the conjugates of 2's and 1's
+    assert list(w) == [0, 1, 2, -1, -1]
+    own = np.flatnonzero(w >= 0)
+    assert relmax(G.to_row_major(F), Fh[own]) < 1e-12
+    assert relmax(G.to_row_major(Wop), Wo[own]) < 1e-12
+    assert relmax(G.to_row_major(Wback), Wb[own]) < 1e-12
+    # what the plan relies on, from the host's own construction of every frequency: the partner's matrices are the
+    # conjugates of the first's
+    for i, j in ((1, 4), (2, 3)):
+        for T in (Fh, Wo, Wb):
+            assert relmax(np.conj(T[i]), T[j]) < 1e-12
+    # without keep_F no F is returned, and the tables are the same
+    F2, Wop2, Wback2, w2 = G.dense_setup(n, z, ks, tau, kind)
+    assert F2 is None and list(w2) == list(w)
+    assert relmax(G.to_row_major(Wop2), Wo[own]) < 1e-12 and relmax(G.to_row_major(Wback2), Wb[own]) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["hybrid", "lp"])
+def test_one_table_per_conjugate_pair(gpu, monkeypatch, kind):
+    """Plans on one pass-matrix table per conjugate pair (the default) agree with plans handed every frequency's own
+    table, and hold about half the table bytes."""
+    G = pkg("gpusetup")
+    I = pkg("inverses")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    n, N = 7, 128
+    x = ph.ellipses(3, N)
+    b = np.stack([R.adrt(x), 0.5 * R.adrt(ph.ellipses(4, N))])
+    cls = I.FastHybridHip if kind == "hybrid" else I.FastLpHip
+    outer = (0.16, 0.96, 12) if kind == "hybrid" else (1.0, 4.0, 12)
+    out, nbytes, slots = [], [], G.table_slots
+    try:
+        for every in (False, True):
+            G.clear_tables()
+            if every:
+                monkeypatch.setattr(G, "table_slots", lambda z, ks: np.arange(len(ks), dtype=np.int32))
+            H = cls(n, dense_cap=10, outer=outer, max_batch=2)
+            assert H.eng.n_dense > 0
+            out.append(H.apply_batch(b))
+            nbytes.append(H.table_bytes)
+            ks, z = np.array(H.eng.dense_k), np.asarray(H.z)
+            del H
+    finally:
+        G.clear_tables()
+    assert relmax(out[0], out[1]) < 1e-13
+    per = 2 * N * N * 16  # W_op and W_back of one frequency
+    assert nbytes[1] == per * ks.size
+    assert nbytes[0] == per * int(np.sum(slots(z, ks) >= 0)) < 0.6 * nbytes[1]
