@@ -186,7 +186,9 @@ def _dense_desc(L, n, ks, w, with_w=True):
     N = 1 << n
     z = np.exp(1j * 2 * np.pi * np.fft.fftfreq(2 * N))
     d, keep = desc(L, n=n, z=None)
-    d.z = L.ptr(z)
+    deg, ac = np.ones(2 * N, dtype=np.int32), np.ones(2 * N)
+    d.z, d.deg, d.a, d.c = L.ptr(z), L.ptr(deg), L.ptr(ac), L.ptr(ac)
+    keep = (keep, deg, ac)
     kk = np.asarray(ks, dtype=np.int32)
     ww = None if w is None else np.asarray(w, dtype=np.int32)
     d.n_dense, d.dense_k, d.dense_w = len(ks), L.ptr(kk), L.ptr(ww)
@@ -223,3 +225,22 @@ def test_table_slots_pairs_conjugate_frequencies():
     z2[63] = np.conj(z[1]) * (1 + 1e-16j)
     z2[63] = complex(z2[63].real, np.nextafter(z2[63].imag, 0.0))
     assert list(G.table_slots(z2, [1, 63])) == [0, 1]
+
+
+def test_plan_bytes_counts_the_tables_given(L):
+    """adrt_plan_bytes (host arithmetic only) charges copied tables: F per dense frequency when given, and the two
+    pass matrices per table, max(dense_w) + 1 of them with dense_w, else one per dense frequency."""
+    n, N = 4, 16
+    lib = L.load()
+    per = N * N * 16
+    base, keep = _dense_desc(L, n, [], None, with_w=False)
+    b0 = lib.adrt_plan_bytes(ctypes.byref(base))
+    cases = [([1, 31, 5], [0, -1, 1], False, 2 * 2), ([1, 31, 5], None, False, 3 * 2), ([1, 31, 5], None, True, 3 * 3),
+             ([1, 31, 5], [0, -1, 1], True, 3 + 2 * 2)]
+    for ks, w, with_f, tables in cases:
+        d, keep = _dense_desc(L, n, ks, w)
+        d.dense_F = 4096 if with_f else None
+        assert lib.adrt_plan_bytes(ctypes.byref(d)) - b0 == tables * per, (ks, w, with_f)
+    d, keep = _dense_desc(L, n, [1, 31, 5], [0, -1, 1])
+    d.dense_on_device = 2  # borrowed: not the plan's bytes
+    assert lib.adrt_plan_bytes(ctypes.byref(d)) == b0
