@@ -86,6 +86,7 @@ class _PlanBase:
         d.dense_w = L.ptr(w)
         d.g1_fp64 = int(bool(getattr(self, "g1_fp64", False)))
         d.l1_fp64 = int(bool(getattr(self, "l1_fp64", False)))
+        self._desc = d  # the plan's descriptor, kept current by set_outer (its arrays are held in _keep)
         self.plan_bytes = int(L.load().adrt_plan_bytes(ctypes.byref(d)))
         h = L.P()
         ctx = getattr(self, "ctx", None)
@@ -102,6 +103,7 @@ class _PlanBase:
     def set_outer(self, lo, hi, D):
         self.lo, self.hi, self.outer_deg = float(lo), float(hi), int(D)
         L.check(L.load().adrt_plan_set_outer(self._plan, self.lo, self.hi, self.outer_deg), "plan_set_outer")
+        self._desc.lo, self._desc.hi, self._desc.outer_deg = self.lo, self.hi, self.outer_deg
 
     # ---- per-pass level-row counts (added to eng.count like the reference's engine does)
     def _count(self, ops, S=1):

@@ -157,3 +157,50 @@ def test_one_table_per_conjugate_pair(gpu, monkeypatch, kind):
     per = 2 * N * N * 16  # W_op and W_back of one frequency
     assert nbytes[1] == per * ks.size
     assert nbytes[0] == per * int(np.sum(slots(z, ks) >= 0)) < 0.6 * nbytes[1]
+
+
+@pytest.mark.parametrize("on_device", [0, 1])
+@pytest.mark.parametrize("compact", [False, True])
+@pytest.mark.parametrize("max_batch", [2, 20])
+def test_plan_copies_host_or_device_tables(gpu, on_device, compact, max_batch):
+    """adrt_plan_create copying its pass matrices, host row-major (dense_on_device 0) or device column-major (1),
+    every frequency's own (dense_w NULL) or one per conjugate pair (dense_w), built on the host by eigh
+    (fastops.py:28-39), gives the results of the package's plan on its borrowed GPU-built tables.  At max_batch 2
+    conjugate frequencies share GEMM units, at 20 (40 rows per frequency) each goes alone."""
+    E = pkg("engine")
+    G = pkg("gpusetup")
+    I = pkg("inverses")
+    L = pkg("lib")
+    ph = pkg("phantom")
+    R = pkg("radon")
+    n, N = 6, 64
+    H = I.FastHybridHip(n, dense_cap=8, outer=(0.16, 0.96, 30), max_batch=max_batch)
+    B = np.stack([R.adrt(ph.ellipses(s, N)) for s in range(3)])
+    ref = H.apply_batch(B)
+    ks = np.asarray(H.eng.dense_k)
+    Fh = E.dense_tables(n, H.z, H.tau, ks, 2, n)
+    Wo, Wb = E.pass_matrices(n, H.z, ks, Fh, "hybrid")
+    w = G.table_slots(H.z, ks) if compact else None
+    if compact:
+        assert 0 < int(np.sum(w >= 0)) < ks.size
+        Wo, Wb = Wo[w >= 0], Wb[w >= 0]
+    if on_device:
+        tabs = [L.DeviceArray.from_host(np.ascontiguousarray(T.transpose(0, 2, 1))) for T in (Wo, Wb)]
+    else:
+        tabs = [np.ascontiguousarray(T) for T in (Wo, Wb)]
+    d = L.PlanDesc.from_buffer_copy(H._desc)
+    d.dense_F = None  # not read by a plan with pass matrices
+    d.dense_Wop, d.dense_Wback = (L.ptr(t) for t in tabs)
+    d.dense_on_device = on_device
+    d.dense_w = L.ptr(w)
+    h = L.P()
+    L.check(L.load().adrt_plan_create(ctypes.byref(h), ctypes.byref(d)), "plan_create")
+    tabs = None  # the plan holds copies
+    saved = H._plan
+    try:
+        H._plan = h.value
+        out = H.apply_batch(B)
+    finally:
+        H._plan = saved
+        L.load().adrt_plan_destroy(h.value)
+    assert relmax(out, ref) < 1e-12
