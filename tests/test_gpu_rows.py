@@ -159,10 +159,11 @@ def test_one_table_per_conjugate_pair(gpu, monkeypatch, kind):
     assert nbytes[0] == per * int(np.sum(slots(z, ks) >= 0)) < 0.6 * nbytes[1]
 
 
+@pytest.mark.parametrize("kind", ["hybrid", "lp"])
 @pytest.mark.parametrize("on_device", [0, 1])
 @pytest.mark.parametrize("compact", [False, True])
 @pytest.mark.parametrize("max_batch", [2, 20])
-def test_plan_copies_host_or_device_tables(gpu, on_device, compact, max_batch):
+def test_plan_copies_host_or_device_tables(gpu, kind, on_device, compact, max_batch):
     """adrt_plan_create copying its pass matrices, host row-major (dense_on_device 0) or device column-major (1),
     every frequency's own (dense_w NULL) or one per conjugate pair (dense_w), built on the host by eigh
     (fastops.py:28-39), gives the results of the package's plan on its borrowed GPU-built tables.  At max_batch 2
@@ -174,12 +175,15 @@ def test_plan_copies_host_or_device_tables(gpu, on_device, compact, max_batch):
     ph = pkg("phantom")
     R = pkg("radon")
     n, N = 6, 64
-    H = I.FastHybridHip(n, dense_cap=8, outer=(0.16, 0.96, 30), max_batch=max_batch)
+    if kind == "hybrid":
+        H = I.FastHybridHip(n, dense_cap=8, outer=(0.16, 0.96, 30), max_batch=max_batch)
+    else:
+        H = I.FastLpHip(n, dense_cap=8, outer=(1.0, 4.0, 25), max_batch=max_batch)
     B = np.stack([R.adrt(ph.ellipses(s, N)) for s in range(3)])
     ref = H.apply_batch(B)
     ks = np.asarray(H.eng.dense_k)
-    Fh = E.dense_tables(n, H.z, H.tau, ks, 2, n)
-    Wo, Wb = E.pass_matrices(n, H.z, ks, Fh, "hybrid")
+    Fh = E.dense_tables(n, H.z, H.tau, ks, 2 if kind == "hybrid" else 1, n)
+    Wo, Wb = E.pass_matrices(n, H.z, ks, Fh, kind)
     w = G.table_slots(H.z, ks) if compact else None
     if compact:
         assert 0 < int(np.sum(w >= 0)) < ks.size
